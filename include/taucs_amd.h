@@ -1,0 +1,106 @@
+/*
+ * taucs_amd.h -- additive entry points of the MI355X multifrontal Cholesky.
+ *
+ * None of these replace a reference function; they extend the TAUCS ABI
+ * (include/taucs.h) without changing any existing signature:
+ *   - seeded generators for the variable-coefficient and shifted workloads,
+ *   - factor statistics and a structure export for parity tests,
+ *   - multi-right-hand-side and device-pointer solves,
+ *   - reduced-precision factors with FP64 iterative refinement,
+ *   - many independent systems factored in the same level launches,
+ *   - a serving loop with HBM-budget admission and resident factors.
+ */
+#ifndef TAUCS_AMD_H
+#define TAUCS_AMD_H
+
+#include <stdint.h>
+#include "taucs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- seeded inputs ------------------------------------------------------ */
+uint64_t taucs_amd_splitmix64(uint64_t* state);
+double   taucs_amd_uniform01(uint64_t* state);
+void     taucs_amd_random_uniform(int n, uint64_t seed, double lo, double hi, double* out);
+/* 7-point operator on the mesh3d pattern and entry order; seed 0 gives
+   exactly taucs_ccs_generate_mesh3d. */
+taucs_ccs_matrix* taucs_amd_generate_mesh3d_varcoef(int X, int Y, int Z, uint64_t seed,
+                                                    double log10_lo, double log10_hi,
+                                                    double shift);
+/* mesh2d(n,"neumann") + i.i.d. uniform[0,1) on each diagonal entry */
+taucs_ccs_matrix* taucs_amd_generate_mesh2d_shifted(int n, uint64_t seed);
+
+/* ---- factor inspection -------------------------------------------------- */
+typedef struct {
+  int     n;
+  int     n_sn;          /* supernode slots, dead merged slots included */
+  int     n_live;        /* supernodes reachable from the root */
+  int     n_levels;      /* height of the supernodal tree + 1 */
+  int     max_front;     /* max (sn_size + update size) */
+  int     etree_depth;
+  int     precision;     /* 0 fp64, 1 fp32, 2 bf16-update */
+  int     has_numeric;
+  double  fund_nnz, fund_flops;
+  double  relax_nnz, relax_flops;
+  double  t_symbolic, t_numeric;   /* seconds, host wall clock */
+  int64_t L_bytes, U_bytes;        /* device storage */
+} taucs_amd_factor_info;
+
+int taucs_amd_factor_get_info(void* L, taucs_amd_factor_info* info);
+
+/* Copy the supernodal structure out.  Arrays sized n_sn+1 (sizes, tree) and
+   struct_ptr n_sn+1, struct_idx sum of sn_up_size. Any pointer may be NULL. */
+int taucs_amd_factor_get_supernodes(void* L, int* sn_size, int* sn_up_size,
+                                    int* first_child, int* next_child,
+                                    int* struct_ptr, int* struct_idx);
+
+/* ---- solves ------------------------------------------------------------- */
+/* X, B column-major host arrays (n x nrhs); permuted space like
+   taucs_supernodal_solve_llt. */
+int taucs_amd_supernodal_solve_many(void* L, int nrhs, double* X, int ldx,
+                                    const double* B, int ldb);
+/* Same, device pointers; runs on the factor's stream and returns when done. */
+int taucs_amd_supernodal_solve_many_device(void* L, int nrhs, double* dX, int ldx,
+                                           const double* dB, int ldb);
+
+/* ---- precision and refinement ------------------------------------------- */
+#define TAUCS_AMD_FP64    0
+#define TAUCS_AMD_FP32    1
+#define TAUCS_AMD_BF16UPD 2
+void* taucs_amd_factor_llt_mf_precision(taucs_ccs_matrix* A, int precision);
+int   taucs_amd_factor_llt_numeric_precision(taucs_ccs_matrix* A, void* L, int precision);
+
+typedef struct {
+  int    iterations;     /* outer iterations used (max over right-hand sides) */
+  int    converged;      /* all right-hand sides reached the tolerance */
+  double backward_error; /* max over rhs of |b-Ax|_inf / (|A|_inf |x|_inf + |b|_inf) */
+  double rel_residual;   /* max over rhs of |b-Ax|_2 / |b|_2 */
+} taucs_amd_refine_info;
+
+/* Solve A X = B (A = the permuted matrix the factor was built from) with
+   FP64 residuals on the device and the factor as preconditioner.
+   method: 0 = classical refinement, 1 = preconditioned conjugate gradients. */
+int taucs_amd_solve_refined(void* L, taucs_ccs_matrix* A, int nrhs, double* X, int ldx,
+                            const double* B, int ldb, int method, double tol, int maxit,
+                            taucs_amd_refine_info* info);
+
+/* ---- timing of the last numeric factorization (device events) ----------- */
+typedef struct {
+  double total_ms;       /* numeric factorization, stream time */
+  double front_ms;       /* dense front kernels (POTRF/TRSM/GEMM/SYRK) */
+  double assembly_ms;    /* A scatter + extend-add */
+  double gemm_ms;        /* tile GEMM/SYRK kernels only */
+  int    gemm_launches;
+  double gemm_flops;     /* executed tile-GEMM flops */
+  double assembly_bytes; /* algorithmic bytes of A scatter + extend-add */
+  int    launches;
+} taucs_amd_numeric_timing;
+void taucs_amd_set_profiling(int on);
+int  taucs_amd_get_numeric_timing(void* L, taucs_amd_numeric_timing* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

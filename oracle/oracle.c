@@ -1,0 +1,555 @@
+/*
+ * ORACLE -- test infrastructure only.  A plain-C restatement of the TAUCS
+ * 2.2.1 symbolic path, used by tests/, __graft_entry__.smoke() and bench.py's
+ * CPU baseline as the checker of libtaucs_amd.so.  Nothing in the product
+ * links or calls this file.
+ *
+ * It follows the reference's own (recursive) formulation function by
+ * function, so that the product's iterative implementation is checked against
+ * an independent restatement:
+ *
+ *   oracle_mesh2d / oracle_mesh3d     src/taucs_ccs_generators.c:89-195, 197-261
+ *   oracle_genmmd                     src/taucs_ccs_order.c:262-442 (adjacency
+ *                                     build, delta=1, maxint=2^31-1) calling
+ *                                     GENMMD itself: external/src/genmmd.c,
+ *                                     compiled unmodified into _ref/ by the
+ *                                     Makefile and loaded with dlopen
+ *   oracle_permute                    src/taucs_ccs_ops.c:174-257
+ *   oracle_etree                      src/taucs_sn_llt.c:3243-3340 (union-find)
+ *   oracle_symbolic                   src/taucs_sn_llt.c:3663-4001 with
+ *                                     recursive_postorder :1269-1287,
+ *                                     recursive_symbolic_elimination :1377-1517,
+ *                                     recursive_amalgamate_supernodes :1527-1786
+ *
+ * Parity pins (see tests/test_oracle.py): GENMMD is the reference binary
+ * itself; the generators are pinned by closed forms (row sums, counts, entry
+ * order); the etree and column counts by brute-force symbolic Cholesky on
+ * small matrices; the fundamental supernodes by their definition (chains with
+ * nested structure); amalgamation by re-deriving every merge decision from
+ * the published thresholds on the final partition.  The reference's own
+ * tests hold no golden vectors for this path.
+ *
+ * The recursion depth equals the elimination-tree height, so the symbolic
+ * entry point runs on a thread with a 1 GiB stack.
+ */
+#define _GNU_SOURCE
+#include <dlfcn.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- generators ----------------------------------------------------------- */
+
+/* 5-point, lower CSC; per column (x,y+1), (x+1,y), diag; Neumann diagonal =
+   degree, +1 at (0,0).  Arrays are malloc'ed; caller frees with free(). */
+int oracle_mesh2d(int n, int** colptr, int** rowind, double** vals)
+{
+  int N = n * n, x, y, ip = 0;
+  int* cp = malloc(sizeof(int) * (N + 1));
+  int* ri = malloc(sizeof(int) * 3 * N);
+  double* v = malloc(sizeof(double) * 3 * N);
+  for (y = 0; y < n; y++)
+    for (x = 0; x < n; x++) {
+      int j = x + y * n;
+      double d = 0;
+      cp[j] = ip;
+      if (y < n - 1) { ri[ip] = x + (y + 1) * n; v[ip++] = -1.0; }
+      if (x < n - 1) { ri[ip] = (x + 1) + y * n; v[ip++] = -1.0; }
+      if (x > 0) d += 1.0;
+      if (y > 0) d += 1.0;
+      if (x < n - 1) d += 1.0;
+      if (y < n - 1) d += 1.0;
+      if (x == 0 && y == 0) d += 1.0;
+      ri[ip] = j;
+      v[ip++] = d;
+    }
+  cp[N] = ip;
+  *colptr = cp; *rowind = ri; *vals = v;
+  return ip;
+}
+
+/* 7-point; per column x+1, y+1, z+1, diag; diagonal = degree, +1 at origin */
+int oracle_mesh3d(int X, int Y, int Z, int** colptr, int** rowind, double** vals)
+{
+  int N = X * Y * Z, x, y, z, ip = 0;
+  int* cp = malloc(sizeof(int) * (N + 1));
+  int* ri = malloc(sizeof(int) * 4 * N);
+  double* v = malloc(sizeof(double) * 4 * N);
+  for (z = 0; z < Z; z++)
+    for (y = 0; y < Y; y++)
+      for (x = 0; x < X; x++) {
+        int j = z * X * Y + y * X + x;
+        double d = 0;
+        cp[j] = ip;
+        if (x < X - 1) { ri[ip] = j + 1; v[ip++] = -1.0; }
+        if (y < Y - 1) { ri[ip] = j + X; v[ip++] = -1.0; }
+        if (z < Z - 1) { ri[ip] = j + X * Y; v[ip++] = -1.0; }
+        if (x < X - 1) d += 1.0;
+        if (y < Y - 1) d += 1.0;
+        if (z < Z - 1) d += 1.0;
+        if (x > 0) d += 1.0;
+        if (y > 0) d += 1.0;
+        if (z > 0) d += 1.0;
+        if (x == 0 && y == 0 && z == 0) d += 1.0;
+        ri[ip] = j;
+        v[ip++] = d;
+      }
+  cp[N] = ip;
+  *colptr = cp; *rowind = ri; *vals = v;
+  return ip;
+}
+
+/* ---- GENMMD through the reference's own compiled routine ------------------ */
+
+/* genmmd.c is the f2c translation with `integer` = long; the routine is
+   called with long arrays so no header of the reference needs changing. */
+typedef int (*genmmd_fn)(long*, long*, long*, long*, long*, long*, long*, long*, long*, long*,
+                         long*, long*);
+static genmmd_fn ref_genmmd = NULL;
+
+int oracle_load_ref(const char* so_path)
+{
+  void* h = dlopen(so_path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return -1;
+  ref_genmmd = (genmmd_fn)dlsym(h, "genmmd_");
+  return ref_genmmd ? 0 : -1;
+}
+
+/* src/taucs_ccs_order.c:262-442: 1-based adjacency, both directions of each
+   off-diagonal entry appended in column-scan order. */
+int oracle_genmmd(int n, const int* colptr, const int* rowind, int* perm, int* invperm)
+{
+  long nn = n, delta = 1, maxint = 2147483647, nofsub = 0;
+  long nnz = colptr[n];
+  long *xadj, *adj, *invp, *prm, *dhead, *qsize, *llist, *marker, *len, *next;
+  long i, j, ip;
+  if (!ref_genmmd) return -1;
+  xadj = calloc(n + 1, sizeof(long));
+  adj = calloc(2 * nnz - n + 1, sizeof(long));
+  invp = calloc(n + 1, sizeof(long));
+  prm = calloc(n + 1, sizeof(long));
+  dhead = calloc(n + 1, sizeof(long));
+  qsize = calloc(n + 1, sizeof(long));
+  llist = calloc(n + 1, sizeof(long));
+  marker = calloc(n + 1, sizeof(long));
+  len = dhead;
+  next = qsize;
+  for (j = 0; j < n; j++)
+    for (ip = colptr[j]; ip < colptr[j + 1]; ip++) {
+      i = rowind[ip];
+      if (i != j) { len[i]++; len[j]++; }
+    }
+  xadj[0] = 1;
+  for (i = 1; i <= n; i++) xadj[i] = xadj[i - 1] + len[i - 1];
+  for (i = 0; i < n; i++) next[i] = xadj[i] - 1;
+  for (j = 0; j < n; j++)
+    for (ip = colptr[j]; ip < colptr[j + 1]; ip++) {
+      i = rowind[ip];
+      if (i != j) {
+        adj[next[i]++] = j + 1;
+        adj[next[j]++] = i + 1;
+      }
+    }
+  ref_genmmd(&nn, xadj, adj, invp, prm, &delta, dhead, qsize, llist, marker, &maxint, &nofsub);
+  for (i = 0; i < n; i++) perm[i] = (int)prm[i] - 1;
+  for (i = 0; i < n; i++) invperm[perm[i]] = (int)i;
+  free(xadj); free(adj); free(invp); free(prm); free(dhead); free(qsize); free(llist); free(marker);
+  return 0;
+}
+
+/* ---- symmetric permutation ------------------------------------------------ */
+
+void oracle_permute(int n, const int* colptr, const int* rowind, const double* vals,
+                    const int* invperm, int* ocolptr, int* orowind, double* ovals)
+{
+  int j, ip;
+  int* len = calloc(n + 1, sizeof(int));
+  for (j = 0; j < n; j++)
+    for (ip = colptr[j]; ip < colptr[j + 1]; ip++) {
+      int I = invperm[rowind[ip]], J = invperm[j];
+      if (I < J) { int t = I; I = J; J = t; }
+      len[J]++;
+    }
+  ocolptr[0] = 0;
+  for (j = 1; j <= n; j++) ocolptr[j] = ocolptr[j - 1] + len[j - 1];
+  for (j = 0; j < n; j++) len[j] = ocolptr[j];
+  for (j = 0; j < n; j++)
+    for (ip = colptr[j]; ip < colptr[j + 1]; ip++) {
+      int I = invperm[rowind[ip]], J = invperm[j];
+      if (I < J) { int t = I; I = J; J = t; }
+      orowind[len[J]] = I;
+      ovals[len[J]] = vals[ip];
+      len[J]++;
+    }
+  free(len);
+}
+
+/* ---- elimination tree (union-find over the row structure) ----------------- */
+
+static int uf_find(int* uf, int i)
+{
+  if (uf[i] != i) uf[i] = uf_find(uf, uf[i]);
+  return uf[i];
+}
+static int uf_union(int* uf, int s, int t)
+{
+  uf[uf_find(uf, s)] = uf_find(uf, t);
+  return uf_find(uf, t);
+}
+
+void oracle_etree(int n, const int* colptr, const int* rowind, int* parent)
+{
+  int nnz = colptr[n], i, j, ip, kp;
+  int* uf = malloc(sizeof(int) * (n + 1));
+  int* rowcount = calloc(n + 1, sizeof(int));
+  int* rowptr = malloc(sizeof(int) * (n + 1));
+  int* colind = malloc(sizeof(int) * (nnz + 1));
+  int* realroot;
+  for (j = 0; j < n; j++)
+    for (ip = colptr[j]; ip < colptr[j + 1]; ip++)
+      if (j < rowind[ip]) rowcount[rowind[ip]]++;
+  ip = 0;
+  for (i = 0; i <= n; i++) {
+    int nx = ip + rowcount[i];
+    rowcount[i] = ip;
+    rowptr[i] = ip;
+    ip = nx;
+  }
+  for (j = 0; j < n; j++)
+    for (ip = colptr[j]; ip < colptr[j + 1]; ip++) {
+      i = rowind[ip];
+      if (i == j) continue;
+      colind[rowcount[i]++] = j;
+    }
+  realroot = rowcount;
+  for (i = 0; i < n; i++) {
+    int vroot = i;
+    uf[i] = i;
+    realroot[i] = i;
+    parent[i] = n;
+    for (kp = rowptr[i]; kp < rowptr[i + 1]; kp++) {
+      int u = uf_find(uf, colind[kp]);
+      int t = realroot[u];
+      if (parent[t] == n && t != i) {
+        parent[t] = i;
+        vroot = uf_union(uf, vroot, u);
+        realroot[vroot] = i;
+      }
+    }
+  }
+  free(uf); free(rowcount); free(rowptr); free(colind);
+}
+
+/* ---- symbolic elimination ------------------------------------------------- */
+
+typedef struct {
+  int n, n_sn, depth;
+  int *sn_size, *sn_up_size, *first_child, *next_child;
+  int** sn_struct;
+  double fund_nnz, fund_flops, relax_nnz, relax_flops;
+  int failed;
+} oracle_sym;
+
+typedef struct {
+  const int *colptr, *rowind;
+  int n;
+  int *first_child, *next_child, *col2sn, *map, *rowind_buf;
+  oracle_sym* L;
+} sym_ctx;
+
+static int rec_symbolic(sym_ctx* c, int j)
+{
+  oracle_sym* L = c->L;
+  int ch, ip, i, csn, in_prev, nnz = 0;
+  for (ch = c->first_child[j]; ch != -1; ch = c->next_child[ch])
+    if (rec_symbolic(c, ch) == -1) return -1;
+
+  in_prev = 1;
+  if (j == c->n) in_prev = 0;
+  else if (c->first_child[j] == -1) in_prev = 0;
+  else if (c->next_child[c->first_child[j]] != -1) in_prev = 0;
+  else {
+    ch = c->first_child[j];
+    for (ip = c->colptr[j]; ip < c->colptr[j + 1]; ip++)
+      in_prev = in_prev && (c->map[c->rowind[ip]] == ch);
+  }
+
+  if (in_prev) {
+    ch = c->first_child[j];
+    csn = c->col2sn[ch];
+    c->col2sn[j] = csn;
+    for (ip = L->sn_size[csn]; ip < L->sn_up_size[csn]; ip++)
+      if (L->sn_struct[csn][ip] == j) break;
+    L->sn_struct[csn][ip] = L->sn_struct[csn][L->sn_size[csn]];
+    L->sn_struct[csn][L->sn_size[csn]] = j;
+    for (ip = L->sn_size[csn]; ip < L->sn_up_size[csn]; ip++) c->map[L->sn_struct[csn][ip]] = j;
+    L->sn_size[csn]++;
+    return 0;
+  }
+
+  if (j < c->n) {
+    nnz = 1;
+    c->rowind_buf[0] = j;
+    c->map[j] = j;
+    for (ch = c->first_child[j]; ch != -1; ch = c->next_child[ch]) {
+      csn = c->col2sn[ch];
+      for (ip = L->sn_size[csn]; ip < L->sn_up_size[csn]; ip++) {
+        i = L->sn_struct[csn][ip];
+        if (i > j && c->map[i] != j) { c->map[i] = j; c->rowind_buf[nnz++] = i; }
+      }
+    }
+    for (ip = c->colptr[j]; ip < c->colptr[j + 1]; ip++) {
+      i = c->rowind[ip];
+      if (c->map[i] != j) { c->map[i] = j; c->rowind_buf[nnz++] = i; }
+    }
+  }
+
+  for (ch = c->first_child[j]; ch != -1; ch = c->next_child[ch]) {
+    csn = c->col2sn[ch];
+    if (ch == c->first_child[j]) L->first_child[L->n_sn] = csn;
+    else {
+      L->next_child[csn] = L->first_child[L->n_sn];
+      L->first_child[L->n_sn] = csn;
+    }
+  }
+
+  if (j < c->n) {
+    c->col2sn[j] = L->n_sn;
+    L->sn_size[L->n_sn] = 1;
+    L->sn_up_size[L->n_sn] = nnz;
+    L->sn_struct[L->n_sn] = malloc(sizeof(int) * nnz);
+    memcpy(L->sn_struct[L->n_sn], c->rowind_buf, sizeof(int) * nnz);
+    L->n_sn++;
+  }
+  return 0;
+}
+
+typedef struct { double zeros, nonzeros; } znz;
+
+static znz rec_amalgamate(sym_ctx* c, int sn)
+{
+  oracle_sym* L = c->L;
+  int i, ip, csn, gc, nnz, nch = 0, new_size, new_up, n = 0;
+  znz own, merged, *cz;
+  double* zcount;
+  own.zeros = 0.0;
+  own.nonzeros = (double)(((L->sn_up_size[sn] - L->sn_size[sn]) * L->sn_size[sn])
+                          + (L->sn_size[sn] * (L->sn_size[sn] + 1)) / 2);
+  if (L->first_child[sn] == -1) return own;
+  for (csn = L->first_child[sn]; csn != -1; csn = L->next_child[csn]) nch++;
+  cz = malloc(sizeof(znz) * nch);
+  i = 0;
+  for (csn = L->first_child[sn]; csn != -1; csn = L->next_child[csn]) cz[i++] = rec_amalgamate(c, csn);
+  merged = own;
+  for (i = 0; i < nch; i++) { merged.nonzeros += cz[i].nonzeros; merged.zeros += cz[i].zeros; }
+  free(cz);
+
+  nnz = 0;
+  for (csn = L->first_child[sn]; csn != -1; csn = L->next_child[csn])
+    for (ip = 0; ip < L->sn_size[csn]; ip++) {
+      i = L->sn_struct[csn][ip];
+      c->map[i] = sn;
+      c->rowind_buf[nnz++] = i;
+    }
+  for (ip = 0; ip < L->sn_size[sn]; ip++) {
+    i = L->sn_struct[sn][ip];
+    c->map[i] = sn;
+    c->rowind_buf[nnz++] = i;
+  }
+  new_size = nnz;
+  for (csn = L->first_child[sn]; csn != -1; csn = L->next_child[csn])
+    for (ip = L->sn_size[csn]; ip < L->sn_up_size[csn]; ip++) {
+      i = L->sn_struct[csn][ip];
+      if (c->map[i] != sn) { c->map[i] = sn; c->rowind_buf[nnz++] = i; }
+    }
+  for (ip = L->sn_size[sn]; ip < L->sn_up_size[sn]; ip++) {
+    i = L->sn_struct[sn][ip];
+    if (c->map[i] != sn) { c->map[i] = sn; c->rowind_buf[nnz++] = i; }
+  }
+  new_up = nnz;
+
+  for (ip = 0; ip < nnz; ip++)
+    if (c->rowind_buf[ip] >= n) n = c->rowind_buf[ip] + 1;
+  zcount = malloc(sizeof(double) * (n + 1));
+  for (ip = 0; ip < new_size; ip++) zcount[c->rowind_buf[ip]] = (double)(ip + 1);
+  for (ip = new_size; ip < new_up; ip++) zcount[c->rowind_buf[ip]] = (double)new_size;
+  for (csn = L->first_child[sn]; csn != -1; csn = L->next_child[csn]) {
+    for (ip = 0; ip < L->sn_size[csn]; ip++) zcount[L->sn_struct[csn][ip]] -= (double)(ip + 1);
+    for (ip = L->sn_size[csn]; ip < L->sn_up_size[csn]; ip++)
+      zcount[L->sn_struct[csn][ip]] -= (double)L->sn_size[csn];
+  }
+  for (ip = 0; ip < L->sn_size[sn]; ip++) zcount[L->sn_struct[sn][ip]] -= (double)(ip + 1);
+  for (ip = L->sn_size[sn]; ip < L->sn_up_size[sn]; ip++)
+    zcount[L->sn_struct[sn][ip]] -= (double)L->sn_size[sn];
+  for (ip = 0; ip < new_up; ip++) merged.zeros += zcount[c->rowind_buf[ip]];
+  free(zcount);
+
+  if (!((new_size < 16)
+        || ((L->sn_size[sn] < 50) && (merged.zeros < 0.5 * merged.nonzeros))
+        || ((L->sn_size[sn] < 250) && (merged.zeros < 0.25 * merged.nonzeros))
+        || ((L->sn_size[sn] < 500) && (merged.zeros < 0.10 * merged.nonzeros))
+        || (merged.zeros < 0.05 * merged.nonzeros)))
+    return own;
+
+  L->sn_size[sn] = new_size;
+  L->sn_up_size[sn] = new_up;
+  L->sn_struct[sn] = realloc(L->sn_struct[sn], sizeof(int) * new_up);
+  memcpy(L->sn_struct[sn], c->rowind_buf, sizeof(int) * new_up);
+
+  nch = 0;
+  for (csn = L->first_child[sn]; csn != -1; csn = L->next_child[csn]) {
+    for (ip = 0; ip < L->sn_size[csn]; ip++) c->col2sn[L->sn_struct[csn][ip]] = sn;
+    for (gc = L->first_child[csn]; gc != -1; gc = L->next_child[gc]) c->rowind_buf[nch++] = gc;
+  }
+  for (csn = L->first_child[sn]; csn != -1; csn = L->next_child[csn]) {
+    free(L->sn_struct[csn]);
+    L->sn_struct[csn] = NULL;
+    L->sn_size[csn] = 0;
+    L->sn_up_size[csn] = 0;
+  }
+  L->first_child[sn] = -1;
+  for (i = 0; i < nch; i++) {
+    L->next_child[c->rowind_buf[i]] = L->first_child[sn];
+    L->first_child[sn] = c->rowind_buf[i];
+  }
+  return merged;
+}
+
+static void stats(oracle_sym* L, double* nnz, double* flops)
+{
+  int sn, i, colnnz;
+  *nnz = 0;
+  *flops = 0;
+  for (sn = 0; sn < L->n_sn; sn++)
+    for (i = 0, colnnz = L->sn_up_size[sn]; i < L->sn_size[sn]; i++, colnnz--) {
+      *flops += 1.0 + (double)colnnz * (double)colnnz;
+      *nnz += (double)colnnz;
+    }
+}
+
+typedef struct {
+  int n;
+  const int *colptr, *rowind;
+  int max_depth;
+  oracle_sym* out;
+} sym_job;
+
+static void* sym_thread(void* arg)
+{
+  sym_job* job = (sym_job*)arg;
+  int n = job->n, j;
+  oracle_sym* L = calloc(1, sizeof(oracle_sym));
+  sym_ctx c;
+  int* parent = malloc(sizeof(int) * (n + 1));
+  job->out = L;
+  L->n = n;
+  L->sn_struct = calloc(n + 1, sizeof(int*));
+  L->sn_size = malloc(sizeof(int) * (n + 1));
+  L->sn_up_size = malloc(sizeof(int) * (n + 1));
+  L->first_child = malloc(sizeof(int) * (n + 1));
+  L->next_child = malloc(sizeof(int) * (n + 1));
+  c.colptr = job->colptr;
+  c.rowind = job->rowind;
+  c.n = n;
+  c.L = L;
+  c.first_child = malloc(sizeof(int) * (n + 1));
+  c.next_child = malloc(sizeof(int) * (n + 1));
+  c.col2sn = malloc(sizeof(int) * (n + 1));
+  c.map = malloc(sizeof(int) * (n + 1));
+  c.rowind_buf = malloc(sizeof(int) * (n + 1));
+
+  oracle_etree(n, job->colptr, job->rowind, parent);
+  for (j = 0; j <= n; j++) c.first_child[j] = -1;
+  for (j = n - 1; j >= 0; j--) {
+    int p = parent[j];
+    c.next_child[j] = c.first_child[p];
+    c.first_child[p] = j;
+  }
+  /* depth by breadth-first levels from the virtual root */
+  {
+    int *cur = malloc(sizeof(int) * (n + 1)), *nxt = malloc(sizeof(int) * (n + 1)), *t;
+    int nc = 1, nn, i;
+    cur[0] = n;
+    L->depth = -1;
+    while (nc) {
+      nn = 0;
+      for (i = 0; i < nc; i++)
+        for (j = c.first_child[cur[i]]; j != -1; j = c.next_child[j]) nxt[nn++] = j;
+      t = cur; cur = nxt; nxt = t;
+      nc = nn;
+      L->depth++;
+    }
+    free(cur);
+    free(nxt);
+  }
+  if (job->max_depth && L->depth > job->max_depth) {
+    L->failed = 1;
+  } else {
+    L->n_sn = 0;
+    for (j = 0; j < n; j++) c.map[j] = -1;
+    for (j = 0; j <= n; j++) L->first_child[j] = L->next_child[j] = -1;
+    rec_symbolic(&c, n);
+    stats(L, &L->fund_nnz, &L->fund_flops);
+    for (j = 0; j < n; j++) c.map[j] = -1;
+    if (L->n_sn > 0) rec_amalgamate(&c, L->n_sn - 1);
+    stats(L, &L->relax_nnz, &L->relax_flops);
+  }
+  free(parent); free(c.first_child); free(c.next_child); free(c.col2sn); free(c.map);
+  free(c.rowind_buf);
+  return NULL;
+}
+
+oracle_sym* oracle_symbolic(int n, const int* colptr, const int* rowind, int max_depth)
+{
+  pthread_t th;
+  pthread_attr_t at;
+  sym_job job = {n, colptr, rowind, max_depth, NULL};
+  pthread_attr_init(&at);
+  pthread_attr_setstacksize(&at, (size_t)1 << 30);
+  if (pthread_create(&th, &at, sym_thread, &job) != 0) return NULL;
+  pthread_join(th, NULL);
+  pthread_attr_destroy(&at);
+  return job.out;
+}
+
+int oracle_sym_info(const oracle_sym* L, int* n_sn, int* depth, int* failed, double* st4)
+{
+  *n_sn = L->n_sn;
+  *depth = L->depth;
+  *failed = L->failed;
+  st4[0] = L->fund_nnz; st4[1] = L->fund_flops; st4[2] = L->relax_nnz; st4[3] = L->relax_flops;
+  return 0;
+}
+
+int oracle_sym_export(const oracle_sym* L, int* sn_size, int* sn_up_size, int* first_child,
+                      int* next_child, int* struct_ptr, int* struct_idx)
+{
+  int sn, p = 0;
+  for (sn = 0; sn <= L->n_sn; sn++) {
+    if (sn_size) sn_size[sn] = sn < L->n_sn ? L->sn_size[sn] : 0;
+    if (sn_up_size) sn_up_size[sn] = sn < L->n_sn ? L->sn_up_size[sn] : 0;
+    if (first_child) first_child[sn] = L->first_child[sn];
+    if (next_child) next_child[sn] = L->next_child[sn];
+  }
+  for (sn = 0; sn < L->n_sn; sn++) {
+    if (struct_ptr) struct_ptr[sn] = p;
+    if (struct_idx && L->sn_up_size[sn] > 0)
+      memcpy(struct_idx + p, L->sn_struct[sn], sizeof(int) * L->sn_up_size[sn]);
+    p += L->sn_up_size[sn];
+  }
+  if (struct_ptr) struct_ptr[L->n_sn] = p;
+  return p;
+}
+
+void oracle_sym_free(oracle_sym* L)
+{
+  int sn;
+  if (!L) return;
+  for (sn = 0; sn < L->n; sn++) free(L->sn_struct[sn]);
+  free(L->sn_struct); free(L->sn_size); free(L->sn_up_size); free(L->first_child);
+  free(L->next_child); free(L);
+}
+
+void oracle_free(void* p) { free(p); }

@@ -1,0 +1,598 @@
+// Host driver of the device numeric phase: builds the task lists of the level
+// schedule once per symbolic factor, owns the HBM arenas, launches the
+// factorization and the solves on the factor's stream.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "kernels.h"
+#include "../host/internal.h"
+#include "../../../include/taucs_amd.h"
+
+namespace tamd {
+template <typename T> __global__ void k_fill(T*, long long);
+template <typename T> __global__ void k_assemble(int, const int*, const long long*, const double*, T*);
+template <typename T> __global__ void k_front_small(const FrontDesc*, const int*, T*, T*, int*);
+template <typename T> __global__ void k_potrf_diag(const FrontDesc*, const PanelTask*, T*, int*);
+template <typename T> __global__ void k_trsm_rows(const FrontDesc*, const PanelTask*, T*);
+template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*);
+__global__ void k_gemm_tile_bf16(const GemmTask*, const float*, float*, float*);
+template <typename T> __global__ void k_extend_add(const FrontDesc*, const EATask*, const int*, T*, T*);
+template <typename T> __global__ void k_solve_init(const FrontDesc*, const int*, const double*, int, T*, int);
+template <typename T> __global__ void k_solve_fwd(const FrontDesc*, const int*, const T*, T*, int);
+template <typename T> __global__ void k_solve_ext(const FrontDesc*, const int*, const int*, T*, int);
+template <typename T> __global__ void k_solve_bwd(const FrontDesc*, const int*, const int*, const T*, T*, double*, int, int);
+}  // namespace tamd
+
+using namespace tamd;
+
+static int g_profiling = 0;
+extern "C" void taucs_amd_set_profiling(int on) { g_profiling = on; }
+
+#define HIPCHECK(x)                                                                  \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) {                                                          \
+      taucs_printf((char*)"taucs_amd: HIP error %s at %s:%d\n", hipGetErrorString(e_), \
+                   __FILE__, __LINE__);                                              \
+      return -1;                                                                     \
+    }                                                                                \
+  } while (0)
+
+struct Op {
+  int kind;
+  int start, count;  // range in the task array of that kind
+  int level;
+};
+
+struct tamd_dev {
+  hipStream_t stream = nullptr;
+  int elem = 8;  // bytes per factor element
+  // structure (uploaded once)
+  FrontDesc* d_fronts = nullptr;
+  int* d_rows = nullptr;
+  int* d_rel = nullptr;
+  int* d_colptr = nullptr;
+  long long* d_adest = nullptr;
+  int* d_small = nullptr;        // front indices for k_front_small
+  PanelTask* d_potrf = nullptr;
+  PanelTask* d_trsm = nullptr;
+  GemmTask* d_gemm = nullptr;
+  EATask* d_ea = nullptr;
+  int* d_solve_list = nullptr;   // fronts by level (solve)
+  int* d_solve_ext = nullptr;    // children by (level, rank) for the solve
+  int* d_info = nullptr;
+  std::vector<Op> ops;
+  std::vector<int> lvl_ptr;      // fronts of level l: [lvl_ptr[l], lvl_ptr[l+1])
+  std::vector<int> sext_ptr;     // solve-ext groups
+  std::vector<int> sext_lvl;
+  int n_fronts = 0, ncol = 0, nnz = 0;
+  long long W_per_rhs = 0;
+  double gemm_flops = 0, assembly_bytes = 0;
+  int gemm_launches = 0;
+  // numeric arenas (precision-dependent)
+  int prec = -1;
+  void* L = nullptr;
+  void* U = nullptr;
+  double* d_avals = nullptr;
+  long long L_elems = 0, U_elems = 0;
+  // solve buffers
+  void* W = nullptr;
+  int W_rhs = 0;
+  double* d_x = nullptr;
+  double* d_b = nullptr;
+  int xb_rhs = 0;
+  taucs_amd_numeric_timing timing;
+};
+
+static int dev_malloc(void** p, size_t bytes)
+{
+  if (tamd_alloc_should_fail()) {
+    *p = nullptr;
+    taucs_printf((char*)"taucs_amd: device allocation failed (induced)\n");
+    return -1;
+  }
+  hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    taucs_printf((char*)"taucs_amd: hipMalloc(%zu) failed: %s\n", bytes, hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+template <typename V>
+static int upload(V** dptr, const std::vector<V>& h, hipStream_t s)
+{
+  if (dev_malloc((void**)dptr, sizeof(V) * (h.size() ? h.size() : 1)) != 0) return -1;
+  if (!h.empty()) HIPCHECK(hipMemcpyAsync(*dptr, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+extern "C" int tamd_dev_available(void)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n > 0;
+}
+
+static void free_ptr(void* p)
+{
+  if (p) (void)hipFree(p);
+}
+
+extern "C" void tamd_dev_free_numeric(tamd_factor* F)
+{
+  tamd_dev* D = F ? F->dev : nullptr;
+  if (!D) return;
+  if (D->stream) (void)hipStreamSynchronize(D->stream);
+  free_ptr(D->L);
+  free_ptr(D->U);
+  free_ptr(D->W);
+  D->L = D->U = D->W = nullptr;
+  D->W_rhs = 0;
+  D->prec = -1;
+}
+
+extern "C" void tamd_refine_forget(void* key);
+
+extern "C" void tamd_dev_free(tamd_factor* F)
+{
+  tamd_dev* D = F ? F->dev : nullptr;
+  if (!D) return;
+  tamd_refine_forget(F);
+  tamd_dev_free_numeric(F);
+  free_ptr(D->d_fronts); free_ptr(D->d_rows); free_ptr(D->d_rel); free_ptr(D->d_colptr);
+  free_ptr(D->d_adest); free_ptr(D->d_small); free_ptr(D->d_potrf); free_ptr(D->d_trsm);
+  free_ptr(D->d_gemm); free_ptr(D->d_ea); free_ptr(D->d_solve_list); free_ptr(D->d_solve_ext);
+  free_ptr(D->d_info); free_ptr(D->d_avals); free_ptr(D->d_x); free_ptr(D->d_b);
+  if (D->stream) (void)hipStreamDestroy(D->stream);
+  delete D;
+  F->dev = nullptr;
+}
+
+// ---- schedule construction --------------------------------------------------
+
+static const int SMALL_M = 64;   // fronts with m <= 64 run whole in one wave
+static const int NB = 64;        // panel block width of large fronts
+
+extern "C" int tamd_dev_setup(tamd_factor* F)
+{
+  const tamd_symbolic* S = &F->sym;
+  const tamd_plan* P = &F->plan;
+  if (!tamd_dev_available()) {
+    taucs_printf((char*)"taucs_amd: no GPU device available; the numeric phase runs only on the GPU\n");
+    return -1;
+  }
+  tamd_dev* D = new tamd_dev();
+  F->dev = D;
+  if (hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking) != hipSuccess) {
+    tamd_dev_free(F);
+    return -1;
+  }
+  const int nf = P->n_live;
+  D->n_fronts = nf;
+  std::vector<int> idx_of(P->n_sn + 1, -1);
+  for (int k = 0; k < nf; k++) idx_of[P->order[k]] = k;
+
+  std::vector<FrontDesc> fr(nf);
+  std::vector<int> rows;
+  long long offW = 0;
+  for (int k = 0; k < nf; k++) {
+    int sn = P->order[k];
+    FrontDesc d;
+    memset(&d, 0, sizeof(d));
+    d.offL = P->off_L[sn];
+    d.offU = P->off_U[sn];
+    d.offW = offW;
+    d.s = S->sn_size[sn];
+    d.m = S->sn_up_size[sn];
+    d.parent = P->sn_parent[sn] < P->n_sn ? idx_of[P->sn_parent[sn]] : -1;
+    d.rel = P->rel_ptr[sn];
+    d.rows = (int)rows.size();
+    rows.insert(rows.end(), S->sn_struct[sn], S->sn_struct[sn] + d.m);
+    offW += d.m;
+    fr[k] = d;
+  }
+  D->W_per_rhs = offW;
+
+  // factorization task lists, level by level
+  std::vector<int> small;
+  std::vector<PanelTask> potrf, trsm;
+  std::vector<GemmTask> gemm;
+  std::vector<EATask> ea;
+  D->lvl_ptr.assign(P->level_ptr, P->level_ptr + P->n_levels + 1);
+  D->ops.clear();
+  D->ops.push_back({TAMD_OP_ASSEMBLE, 0, 0, -1});
+  D->gemm_flops = 0;
+  D->assembly_bytes = 12.0 * P->a_nnz;
+  for (int l = 0; l < P->n_levels; l++) {
+    int b0 = P->level_ptr[l], b1 = P->level_ptr[l + 1];
+    int st = (int)small.size();
+    int maxsteps = 0;
+    for (int k = b0; k < b1; k++) {
+      if (fr[k].m <= SMALL_M) small.push_back(k);
+      else maxsteps = std::max(maxsteps, (fr[k].s + NB - 1) / NB);
+    }
+    if ((int)small.size() > st) D->ops.push_back({TAMD_OP_SMALL, st, (int)small.size() - st, l});
+    for (int step = 0; step < maxsteps; step++) {
+      int p0 = (int)potrf.size(), t0 = (int)trsm.size(), g0 = (int)gemm.size();
+      for (int k = b0; k < b1; k++) {
+        const FrontDesc& d = fr[k];
+        if (d.m <= SMALL_M) continue;
+        int k0 = step * NB;
+        if (k0 >= d.s) continue;
+        int kb = std::min(NB, d.s - k0);
+        potrf.push_back({k, k0, kb, 0, 0});
+        for (int r0 = k0 + kb; r0 < d.m; r0 += 64) trsm.push_back({k, k0, kb, r0, std::min(64, d.m - r0)});
+        for (int c0 = k0 + kb; c0 < d.s; c0 += 64)
+          for (int r0 = c0; r0 < d.m; r0 += 64) {
+            GemmTask g;
+            g.a_off = d.offL + r0 + (long long)k0 * d.m;
+            g.b_off = d.offL + c0 + (long long)k0 * d.m;
+            g.c_off = d.offL + r0 + (long long)c0 * d.m;
+            g.ld = d.m;
+            g.ldc = d.m;
+            g.M = std::min(64, d.m - r0);
+            g.N = std::min(64, d.s - c0);
+            g.K = kb;
+            g.flags = (r0 == c0) ? 2 : 0;
+            gemm.push_back(g);
+            D->gemm_flops += 2.0 * g.M * g.N * g.K;
+          }
+      }
+      if ((int)potrf.size() > p0) D->ops.push_back({TAMD_OP_POTRF, p0, (int)potrf.size() - p0, l});
+      if ((int)trsm.size() > t0) D->ops.push_back({TAMD_OP_TRSM, t0, (int)trsm.size() - t0, l});
+      if ((int)gemm.size() > g0) D->ops.push_back({TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l});
+    }
+    // SYRK of the update blocks, K = s
+    {
+      int g0 = (int)gemm.size();
+      for (int k = b0; k < b1; k++) {
+        const FrontDesc& d = fr[k];
+        int u = d.m - d.s;
+        if (d.m <= SMALL_M || u == 0) continue;
+        for (int j0 = 0; j0 < u; j0 += 64)
+          for (int i0 = j0; i0 < u; i0 += 64) {
+            GemmTask g;
+            g.a_off = d.offL + d.s + i0;
+            g.b_off = d.offL + d.s + j0;
+            g.c_off = d.offU + i0 + (long long)j0 * u;
+            g.ld = d.m;
+            g.ldc = u;
+            g.M = std::min(64, u - i0);
+            g.N = std::min(64, u - j0);
+            g.K = d.s;
+            g.flags = 1 | (i0 == j0 ? 2 : 0);
+            gemm.push_back(g);
+            D->gemm_flops += 2.0 * g.M * g.N * g.K;
+          }
+      }
+      if ((int)gemm.size() > g0) D->ops.push_back({TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l});
+    }
+    // extend-add of this level's fronts into their parents, by sibling rank
+    {
+      int maxrank = -1;
+      for (int k = b0; k < b1; k++)
+        if (fr[k].parent >= 0) maxrank = std::max(maxrank, P->sib_rank[P->order[k]]);
+      for (int rk = 0; rk <= maxrank; rk++) {
+        int e0 = (int)ea.size();
+        for (int k = b0; k < b1; k++) {
+          const FrontDesc& d = fr[k];
+          int u = d.m - d.s;
+          if (d.parent < 0 || u == 0 || P->sib_rank[P->order[k]] != rk) continue;
+          D->assembly_bytes += 24.0 * ((double)u * (u + 1) / 2);
+          // chunks of about 16K entries
+          int j = 0;
+          while (j < u) {
+            int j1 = j;
+            long long acc = 0;
+            while (j1 < u && acc < 16384) { acc += u - j1; j1++; }
+            ea.push_back({k, j, j1, 0});
+            j = j1;
+          }
+        }
+        if ((int)ea.size() > e0) D->ops.push_back({TAMD_OP_EXTEND, e0, (int)ea.size() - e0, l});
+      }
+    }
+  }
+
+  // solve lists: fronts by level, and children by (level, rank)
+  std::vector<int> slist(nf);
+  for (int k = 0; k < nf; k++) slist[k] = k;
+  std::vector<int> sext;
+  D->sext_ptr.clear();
+  D->sext_lvl.clear();
+  for (int l = 0; l < P->n_levels; l++) {
+    int b0 = P->level_ptr[l], b1 = P->level_ptr[l + 1];
+    int maxrank = -1;
+    for (int k = b0; k < b1; k++)
+      if (fr[k].parent >= 0 && fr[k].m > fr[k].s) maxrank = std::max(maxrank, P->sib_rank[P->order[k]]);
+    for (int rk = 0; rk <= maxrank; rk++) {
+      D->sext_ptr.push_back((int)sext.size());
+      D->sext_lvl.push_back(l);
+      for (int k = b0; k < b1; k++)
+        if (fr[k].parent >= 0 && fr[k].m > fr[k].s && P->sib_rank[P->order[k]] == rk) sext.push_back(k);
+    }
+  }
+  D->sext_ptr.push_back((int)sext.size());
+
+  D->ncol = S->n;
+  D->nnz = P->a_nnz;
+  std::vector<long long> adest(P->a_dest, P->a_dest + P->a_nnz);
+  std::vector<int> rel(P->rel, P->rel + (P->rel_ptr[P->n_sn] > 0 ? P->rel_ptr[P->n_sn] : 0));
+  if (upload(&D->d_fronts, fr, D->stream) || upload(&D->d_rows, rows, D->stream)
+      || upload(&D->d_rel, rel, D->stream) || upload(&D->d_adest, adest, D->stream)
+      || upload(&D->d_small, small, D->stream) || upload(&D->d_potrf, potrf, D->stream)
+      || upload(&D->d_trsm, trsm, D->stream) || upload(&D->d_gemm, gemm, D->stream)
+      || upload(&D->d_ea, ea, D->stream) || upload(&D->d_solve_list, slist, D->stream)
+      || upload(&D->d_solve_ext, sext, D->stream)
+      || dev_malloc((void**)&D->d_info, sizeof(int))
+      || dev_malloc((void**)&D->d_colptr, sizeof(int) * (S->n + 1))
+      || dev_malloc((void**)&D->d_avals, sizeof(double) * (P->a_nnz > 0 ? P->a_nnz : 1))) {
+    tamd_dev_free(F);
+    return -1;
+  }
+  if (hipStreamSynchronize(D->stream) != hipSuccess) {
+    tamd_dev_free(F);
+    return -1;
+  }
+  D->L_elems = P->L_elems;
+  D->U_elems = P->U_elems;
+  return 0;
+}
+
+// ---- numeric factorization ------------------------------------------------
+
+static inline unsigned grid_for(long long n, int block)
+{
+  long long g = (n + block - 1) / block;
+  if (g > 65535LL * 64) g = 65535LL * 64;
+  return (unsigned)(g > 0 ? g : 1);
+}
+
+template <typename T>
+static int run_numeric(tamd_dev* D, bool bf16upd)
+{
+  hipStream_t st = D->stream;
+  T* L = (T*)D->L;
+  T* U = (T*)D->U;
+  hipEvent_t e0 = nullptr, e1 = nullptr, a0 = nullptr, a1 = nullptr;
+  const bool prof = g_profiling != 0;
+  memset(&D->timing, 0, sizeof(D->timing));
+  if (prof) {
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventCreate(&a0));
+    HIPCHECK(hipEventCreate(&a1));
+    HIPCHECK(hipEventRecord(e0, st));
+  }
+  hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(D->L_elems, 256)), dim3(256), 0, st, L, D->L_elems);
+  if (D->U_elems > 0)
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(D->U_elems, 256)), dim3(256), 0, st, U, D->U_elems);
+  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, st, D->d_info, 1LL);
+  int launches = 3;
+  for (const Op& op : D->ops) {
+    if (prof) HIPCHECK(hipEventRecord(a0, st));
+    switch (op.kind) {
+      case TAMD_OP_ASSEMBLE:
+        hipLaunchKernelGGL(k_assemble<T>, dim3(grid_for(D->ncol, 128)), dim3(128), 0, st, D->ncol,
+                           D->d_colptr, D->d_adest, D->d_avals, L);
+        break;
+      case TAMD_OP_SMALL:
+        hipLaunchKernelGGL(k_front_small<T>, dim3(op.count), dim3(64), 0, st, D->d_fronts,
+                           D->d_small + op.start, L, U, D->d_info);
+        break;
+      case TAMD_OP_POTRF:
+        hipLaunchKernelGGL(k_potrf_diag<T>, dim3(op.count), dim3(64), 0, st, D->d_fronts,
+                           D->d_potrf + op.start, L, D->d_info);
+        break;
+      case TAMD_OP_TRSM:
+        hipLaunchKernelGGL(k_trsm_rows<T>, dim3(op.count), dim3(64), 0, st, D->d_fronts,
+                           D->d_trsm + op.start, L);
+        break;
+      case TAMD_OP_GEMM:
+        if (bf16upd)
+          hipLaunchKernelGGL(k_gemm_tile_bf16, dim3(op.count), dim3(256), 0, st,
+                             D->d_gemm + op.start, (const float*)L, (float*)L, (float*)U);
+        else
+          hipLaunchKernelGGL(k_gemm_tile<T>, dim3(op.count), dim3(256), 0, st,
+                             D->d_gemm + op.start, (const T*)L, L, U);
+        break;
+      case TAMD_OP_EXTEND:
+        hipLaunchKernelGGL(k_extend_add<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts,
+                           D->d_ea + op.start, D->d_rel, L, U);
+        break;
+    }
+    launches++;
+    if (prof) {
+      float ms = 0;
+      HIPCHECK(hipEventRecord(a1, st));
+      HIPCHECK(hipEventSynchronize(a1));
+      HIPCHECK(hipEventElapsedTime(&ms, a0, a1));
+      if (op.kind == TAMD_OP_ASSEMBLE || op.kind == TAMD_OP_EXTEND) D->timing.assembly_ms += ms;
+      else D->timing.front_ms += ms;
+      if (op.kind == TAMD_OP_GEMM) {
+        D->timing.gemm_ms += ms;
+        D->timing.gemm_launches++;
+      }
+    }
+  }
+  HIPCHECK(hipGetLastError());
+  if (prof) {
+    float ms = 0;
+    HIPCHECK(hipEventRecord(e1, st));
+    HIPCHECK(hipEventSynchronize(e1));
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    D->timing.total_ms = ms;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(a0); (void)hipEventDestroy(a1);
+  }
+  D->timing.launches = launches;
+  D->timing.gemm_flops = D->gemm_flops;
+  D->timing.assembly_bytes = D->assembly_bytes;
+  int info = 0;
+  HIPCHECK(hipMemcpyAsync(&info, D->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  if (info) {
+    taucs_printf((char*)"\t\tLL^T Factorization: Matrix is not positive definite.\n");
+    return -1;
+  }
+  return 0;
+}
+
+extern "C" int tamd_dev_numeric(tamd_factor* F, const taucs_ccs_matrix* A)
+{
+  tamd_dev* D = F->dev;
+  if (!D) return -1;
+  int prec = F->precision;
+  int elem = prec == TAMD_PREC_FP64 ? 8 : 4;
+  if (D->prec != prec) {
+    tamd_dev_free_numeric(F);
+    if (dev_malloc(&D->L, (size_t)elem * (D->L_elems > 0 ? D->L_elems : 1)) != 0
+        || dev_malloc(&D->U, (size_t)elem * (D->U_elems > 0 ? D->U_elems : 1)) != 0) {
+      tamd_dev_free_numeric(F);
+      return -1;
+    }
+    D->prec = prec;
+    D->elem = elem;
+  }
+  tamd_refine_forget(F);
+  HIPCHECK(hipMemcpyAsync(D->d_colptr, A->colptr, sizeof(int) * (A->n + 1), hipMemcpyHostToDevice, D->stream));
+  HIPCHECK(hipMemcpyAsync(D->d_avals, A->values.d, sizeof(double) * (size_t)F->plan.a_nnz,
+                          hipMemcpyHostToDevice, D->stream));
+  int rc;
+  if (prec == TAMD_PREC_FP64) rc = run_numeric<double>(D, false);
+  else rc = run_numeric<float>(D, prec == TAMD_PREC_BF16UPD);
+  return rc;
+}
+
+extern "C" int taucs_amd_get_numeric_timing(void* vL, taucs_amd_numeric_timing* t)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  if (!F || !F->dev || !t) return -1;
+  *t = F->dev->timing;
+  return 0;
+}
+
+// ---- solves ------------------------------------------------------------------
+
+template <typename T>
+static int run_solve(tamd_factor* F, int nrhs, double* dX, int ldx, const double* dB, int ldb)
+{
+  tamd_dev* D = F->dev;
+  const tamd_plan* P = &F->plan;
+  hipStream_t st = D->stream;
+  if (!D->W || D->W_rhs < nrhs) {
+    free_ptr(D->W);
+    D->W = nullptr;
+    int cap = std::max(nrhs, 16);
+    if (dev_malloc(&D->W, sizeof(T) * (size_t)D->W_per_rhs * cap) != 0) return -1;
+    D->W_rhs = cap;
+  }
+  T* W = (T*)D->W;
+  const T* L = (const T*)D->L;
+  hipLaunchKernelGGL(k_solve_init<T>, dim3(D->n_fronts), dim3(128), 0, st, D->d_fronts, D->d_rows,
+                     dB, ldb, W, nrhs);
+  for (int l = 0; l < P->n_levels; l++) {
+    int b0 = D->lvl_ptr[l], b1 = D->lvl_ptr[l + 1];
+    hipLaunchKernelGGL(k_solve_fwd<T>, dim3(b1 - b0), dim3(256), 0, st, D->d_fronts,
+                       D->d_solve_list + b0, L, W, nrhs);
+    for (size_t q = 0; q + 1 < D->sext_ptr.size(); q++) {
+      if (D->sext_lvl[q] != l) continue;
+      int c0 = D->sext_ptr[q], c1 = D->sext_ptr[q + 1];
+      if (c1 > c0)
+        hipLaunchKernelGGL(k_solve_ext<T>, dim3(c1 - c0), dim3(128), 0, st, D->d_fronts,
+                           D->d_solve_ext + c0, D->d_rel, W, nrhs);
+    }
+  }
+  for (int l = P->n_levels - 1; l >= 0; l--) {
+    int b0 = D->lvl_ptr[l], b1 = D->lvl_ptr[l + 1];
+    hipLaunchKernelGGL(k_solve_bwd<T>, dim3(b1 - b0), dim3(256), 0, st, D->d_fronts,
+                       D->d_solve_list + b0, D->d_rows, L, W, dX, ldx, nrhs);
+  }
+  HIPCHECK(hipGetLastError());
+  return 0;
+}
+
+static int solve_device(tamd_factor* F, int nrhs, double* dX, int ldx, const double* dB, int ldb)
+{
+  // at most 16 right-hand sides per sweep (a front's workspace block is
+  // m x nrhs at offset offW * nrhs)
+  tamd_dev* D = F->dev;
+  for (int r0 = 0; r0 < nrhs; r0 += 16) {
+    int nr = std::min(16, nrhs - r0);
+    int rc = (D->prec == TAMD_PREC_FP64)
+                 ? run_solve<double>(F, nr, dX + (size_t)r0 * ldx, ldx, dB + (size_t)r0 * ldb, ldb)
+                 : run_solve<float>(F, nr, dX + (size_t)r0 * ldx, ldx, dB + (size_t)r0 * ldb, ldb);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" int tamd_dev_stream_of(tamd_factor* F, hipStream_t* st)
+{
+  if (!F || !F->dev) return -1;
+  *st = F->dev->stream;
+  return 0;
+}
+
+extern "C" int tamd_dev_solve_device(tamd_factor* F, int nrhs, double* dX, int ldx, const double* dB,
+                                     int ldb)
+{
+  return solve_device(F, nrhs, dX, ldx, dB, ldb);
+}
+
+extern "C" int taucs_amd_supernodal_solve_many_device(void* vL, int nrhs, double* dX, int ldx,
+                                                      const double* dB, int ldb)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  if (!F || !F->dev || !F->has_numeric) return -1;
+  int rc = solve_device(F, nrhs, dX, ldx, dB, ldb);
+  if (rc) return rc;
+  HIPCHECK(hipStreamSynchronize(F->dev->stream));
+  return 0;
+}
+
+extern "C" int tamd_dev_solve(tamd_factor* F, int nrhs, double* X, int ldx, const double* B, int ldb)
+{
+  tamd_dev* D = F->dev;
+  int n = F->n;
+  if (!D || !D->L) return -1;
+  if (!D->d_x || D->xb_rhs < nrhs) {
+    free_ptr(D->d_x);
+    free_ptr(D->d_b);
+    D->d_x = D->d_b = nullptr;
+    if (dev_malloc((void**)&D->d_x, sizeof(double) * (size_t)n * nrhs) != 0
+        || dev_malloc((void**)&D->d_b, sizeof(double) * (size_t)n * nrhs) != 0)
+      return -1;
+    D->xb_rhs = nrhs;
+  }
+  HIPCHECK(hipMemcpy2DAsync(D->d_b, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n,
+                            nrhs, hipMemcpyHostToDevice, D->stream));
+  if (solve_device(F, nrhs, D->d_x, n, D->d_b, n) != 0) return -1;
+  HIPCHECK(hipMemcpy2DAsync(X, sizeof(double) * ldx, D->d_x, sizeof(double) * n, sizeof(double) * n,
+                            nrhs, hipMemcpyDeviceToHost, D->stream));
+  HIPCHECK(hipStreamSynchronize(D->stream));
+  return 0;
+}
+
+extern "C" int tamd_dev_download_blocks(tamd_factor* F, double* Lhost)
+{
+  tamd_dev* D = F->dev;
+  if (!D || !D->L) return -1;
+  size_t n = (size_t)D->L_elems;
+  if (D->prec == TAMD_PREC_FP64) {
+    HIPCHECK(hipMemcpyAsync(Lhost, D->L, sizeof(double) * n, hipMemcpyDeviceToHost, D->stream));
+    HIPCHECK(hipStreamSynchronize(D->stream));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHECK(hipMemcpyAsync(tmp.data(), D->L, sizeof(float) * n, hipMemcpyDeviceToHost, D->stream));
+    HIPCHECK(hipStreamSynchronize(D->stream));
+    for (size_t i = 0; i < n; i++) Lhost[i] = tmp[i];
+  }
+  return 0;
+}

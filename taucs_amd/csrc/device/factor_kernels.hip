@@ -1,0 +1,315 @@
+// Numeric-factorization kernels: A assembly, small fronts, blocked panel
+// POTRF/TRSM, MFMA tile GEMM/SYRK, extend-add.
+//
+// Reference computation per front (src/taucs_sn_llt.c:1086-1194): assemble
+// A's columns into [F1;F2], dpotrf F1, dtrsm F2 <- F2 L11^-T, dsyrk
+// U <- U - F2 F2^T; then extend-add U into the parent (:1200-1240).
+#include "kernels.h"
+
+namespace tamd {
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, long long n)
+{
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = (T)0;
+}
+
+// One thread per A column, entries in stored order: duplicates (if any) are
+// summed deterministically.
+template <typename T>
+__global__ void k_assemble(int ncol, const int* __restrict__ colptr,
+                           const long long* __restrict__ dest,
+                           const double* __restrict__ vals, T* __restrict__ L)
+{
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  int e = colptr[j + 1];
+  for (int ip = colptr[j]; ip < e; ip++) L[dest[ip]] += (T)vals[ip];
+}
+
+// Right-looking unblocked Cholesky of the first `ncols` columns of an
+// nrows x ncols block held in LDS (ld 65), one row per lane.  Rows beyond
+// ncols are the sub-diagonal part and receive the triangular solve.
+// Returns nonzero in *bad when a pivot is not positive.
+template <typename T>
+__device__ void lds_panel_chol(T* P, int nrows, int ncols, int lane, int* bad)
+{
+  for (int k = 0; k < ncols; k++) {
+    if (lane == k) {
+      T d = P[k + k * 65];
+      if (!(d > (T)0)) { *bad = 1; d = (T)1; }
+      P[k + k * 65] = sqrt(d);
+    }
+    __syncthreads();
+    T piv = P[k + k * 65];
+    if (lane > k && lane < nrows) P[lane + k * 65] /= piv;
+    __syncthreads();
+    if (lane > k && lane < nrows) {
+      T lk = P[lane + k * 65];
+      int jmax = lane < ncols ? lane : ncols - 1;
+      for (int j = k + 1; j <= jmax; j++) P[lane + j * 65] -= lk * P[j + k * 65];
+    }
+    __syncthreads();
+  }
+}
+
+// Whole front (m <= 64) in one wave: POTRF + TRSM in LDS, SYRK into U.
+template <typename T>
+__global__ __launch_bounds__(64) void k_front_small(const FrontDesc* __restrict__ fr,
+                                                    const int* __restrict__ list,
+                                                    T* __restrict__ L, T* __restrict__ U,
+                                                    int* __restrict__ info)
+{
+  __shared__ T P[64 * 65];
+  __shared__ int bad;
+  const FrontDesc d = fr[list[blockIdx.x]];
+  const int s = d.s, m = d.m, u = m - s, lane = threadIdx.x;
+  T* gp = L + d.offL;
+  if (lane == 0) bad = 0;
+  for (int c = 0; c < s; c++)
+    if (lane < m) P[lane + c * 65] = gp[lane + (long long)c * m];
+  __syncthreads();
+  lds_panel_chol(P, m, s, lane, &bad);
+  for (int c = 0; c < s; c++)
+    if (lane < m && lane >= c) gp[lane + (long long)c * m] = P[lane + c * 65];
+  if (u > 0 && lane < u) {
+    T* gu = U + d.offU;
+    const int i = lane;
+    for (int j = 0; j <= i; j++) {
+      T acc = (T)0;
+      for (int k = 0; k < s; k++) acc += P[s + i + k * 65] * P[s + j + k * 65];
+      gu[i + (long long)j * u] -= acc;
+    }
+  }
+  if (lane == 0 && bad) atomicOr(info, 1);
+}
+
+// Cholesky of one kb x kb diagonal block of a large front's panel.
+template <typename T>
+__global__ __launch_bounds__(64) void k_potrf_diag(const FrontDesc* __restrict__ fr,
+                                                   const PanelTask* __restrict__ tasks,
+                                                   T* __restrict__ L, int* __restrict__ info)
+{
+  __shared__ T D[64 * 65];
+  __shared__ int bad;
+  const PanelTask t = tasks[blockIdx.x];
+  const FrontDesc d = fr[t.front];
+  const int m = d.m, lane = threadIdx.x, kb = t.kb;
+  T* gp = L + d.offL + t.k0 + (long long)t.k0 * m;
+  if (lane == 0) bad = 0;
+  for (int c = 0; c < kb; c++)
+    if (lane < kb) D[lane + c * 65] = gp[lane + (long long)c * m];
+  __syncthreads();
+  lds_panel_chol(D, kb, kb, lane, &bad);
+  for (int c = 0; c < kb; c++)
+    if (lane < kb && lane >= c) gp[lane + (long long)c * m] = D[lane + c * 65];
+  if (lane == 0 && bad) atomicOr(info, 1);
+}
+
+// Rows r0..r0+nr-1 of columns k0..k0+kb-1:  X <- X * L_kk^{-T}.
+template <typename T>
+__global__ __launch_bounds__(64) void k_trsm_rows(const FrontDesc* __restrict__ fr,
+                                                  const PanelTask* __restrict__ tasks,
+                                                  T* __restrict__ L)
+{
+  __shared__ T Lk[64 * 65];
+  __shared__ T X[64 * 65];
+  const PanelTask t = tasks[blockIdx.x];
+  const FrontDesc d = fr[t.front];
+  const int m = d.m, lane = threadIdx.x, kb = t.kb, nr = t.nr;
+  const T* gl = L + d.offL + t.k0 + (long long)t.k0 * m;
+  T* gx = L + d.offL + t.r0 + (long long)t.k0 * m;
+  for (int c = 0; c < kb; c++) {
+    if (lane < kb) Lk[lane + c * 65] = gl[lane + (long long)c * m];
+    if (lane < nr) X[lane + c * 65] = gx[lane + (long long)c * m];
+  }
+  __syncthreads();
+  if (lane < nr) {
+    for (int j = 0; j < kb; j++) {
+      T v = X[lane + j * 65];
+      for (int q = 0; q < j; q++) v -= X[lane + q * 65] * Lk[j + q * 65];
+      X[lane + j * 65] = v / Lk[j + j * 65];
+    }
+    for (int c = 0; c < kb; c++) gx[lane + (long long)c * m] = X[lane + c * 65];
+  }
+}
+
+// ---- MFMA tile GEMM ---------------------------------------------------------
+// 64x64 tile, 4 waves (2x2), each wave a 32x32 sub-tile of 2x2 16x16 MFMA
+// blocks.  K is staged through LDS in chunks of 16.  The MFMA is issued as
+// D = B_tile * A_tile^T so that a lane's four results are four consecutive
+// columns of one row: the epilogue's read-modify-write of C walks down
+// columns (16 consecutive rows per lane group).
+//
+// Lane layout of v_mfma_*_16x16x4 (checked on gfx950): A operand lane l holds
+// A[l%16][l/16], B operand lane l holds B[l/16][l%16].  Result register r of
+// lane l is D[l/16 + 4r][l%16] for f64 and D[4(l/16) + r][l%16] for f32.
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef d4_t acc_t;
+  static __device__ acc_t step(double a, double b, acc_t c)
+  {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+  typedef f4_t acc_t;
+  static __device__ acc_t step(float a, float b, acc_t c)
+  {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+#define TK 16
+#define TPAD 4
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ tasks,
+                                                   const T* Lsrc, T* L, T* U)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T As[TK][64 + TPAD];
+  __shared__ T Bs[TK][64 + TPAD];
+  const GemmTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const T* A = Lsrc + t.a_off;
+  const T* B = Lsrc + t.b_off;
+  acc_t acc[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
+
+  const int li = tid & 63, lk = tid >> 6;  // loader: row li, k lk + 4q
+  for (int kc = 0; kc < t.K; kc += TK) {
+    for (int q = 0; q < 4; q++) {
+      int k = lk + 4 * q;
+      bool kin = kc + k < t.K;
+      As[k][li] = (kin && li < t.M) ? A[li + (long long)(kc + k) * t.ld] : (T)0;
+      Bs[k][li] = (kin && li < t.N) ? B[li + (long long)(kc + k) * t.ld] : (T)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 4) {
+      const int kr = kk + (lane >> 4), c16 = lane & 15;
+      T a0 = As[kr][wr + c16], a1 = As[kr][wr + 16 + c16];
+      T b0 = Bs[kr][wc + c16], b1 = Bs[kr][wc + 16 + c16];
+      acc[0][0] = Mfma<T>::step(b0, a0, acc[0][0]);
+      acc[0][1] = Mfma<T>::step(b1, a0, acc[0][1]);
+      acc[1][0] = Mfma<T>::step(b0, a1, acc[1][0]);
+      acc[1][1] = Mfma<T>::step(b1, a1, acc[1][1]);
+    }
+    __syncthreads();
+  }
+
+  T* C = ((t.flags & 1) ? U : L) + t.c_off;
+  const bool diag = (t.flags & 2) != 0;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int r = 0; r < 4; r++) {
+        int i = wr + a * 16 + (lane & 15);
+        int j = wc + b * 16 + Mfma<T>::drow(lane, r);
+        if (i < t.M && j < t.N && (!diag || i >= j))
+          C[i + (long long)j * t.ldc] -= acc[a][b][r];
+      }
+}
+
+// Same tile, BF16 inputs with FP32 accumulation (v_mfma_f32_16x16x32_bf16):
+// the update variant of the FP32 factor.  Operands are rounded to bf16 as
+// they are staged into LDS; K chunk 32.
+__global__ __launch_bounds__(256) void k_gemm_tile_bf16(const GemmTask* __restrict__ tasks,
+                                                        const float* Lsrc, float* L,
+                                                        float* U)
+{
+  __shared__ __bf16 As[64][32 + 8];  // [row][k]
+  __shared__ __bf16 Bs[64][32 + 8];
+  const GemmTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const float* A = Lsrc + t.a_off;
+  const float* B = Lsrc + t.b_off;
+  f4_t acc[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int r = 0; r < 4; r++) acc[a][b][r] = 0.f;
+  const int li = tid & 63, lk = tid >> 6;
+  for (int kc = 0; kc < t.K; kc += 32) {
+    for (int q = 0; q < 8; q++) {
+      int k = lk + 4 * q;
+      bool kin = kc + k < t.K;
+      As[li][k] = (__bf16)((kin && li < t.M) ? A[li + (long long)(kc + k) * t.ld] : 0.f);
+      Bs[li][k] = (__bf16)((kin && li < t.N) ? B[li + (long long)(kc + k) * t.ld] : 0.f);
+    }
+    __syncthreads();
+    {
+      const int c16 = lane & 15, kg = (lane >> 4) * 8;
+      bf16x8_t a0, a1, b0, b1;
+      for (int e = 0; e < 8; e++) {
+        a0[e] = As[wr + c16][kg + e];
+        a1[e] = As[wr + 16 + c16][kg + e];
+        b0[e] = Bs[wc + c16][kg + e];
+        b1[e] = Bs[wc + 16 + c16][kg + e];
+      }
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, a0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, a0, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, a1, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, a1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  float* C = ((t.flags & 1) ? U : L) + t.c_off;
+  const bool diag = (t.flags & 2) != 0;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int r = 0; r < 4; r++) {
+        int i = wr + a * 16 + (lane & 15);
+        int j = wc + b * 16 + 4 * (lane >> 4) + r;
+        if (i < t.M && j < t.N && (!diag || i >= j))
+          C[i + (long long)j * t.ldc] -= acc[a][b][r];
+      }
+}
+
+// Extend-add: child update block entries (i >= j) added into the parent's
+// panel or update block at the relative positions, kept in the lower part.
+template <typename T>
+__global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict__ fr,
+                                                    const EATask* __restrict__ tasks,
+                                                    const int* __restrict__ rel,
+                                                    T* __restrict__ L, T* __restrict__ U)
+{
+  const EATask t = tasks[blockIdx.x];
+  const FrontDesc c = fr[t.child];
+  const FrontDesc p = fr[c.parent];
+  const int uc = c.m - c.s, sp = p.s, mp = p.m, up = mp - sp;
+  const T* Uc = U + c.offU;
+  const int* r = rel + c.rel;
+  for (int j = t.j0; j < t.j1; j++) {
+    const int pj = r[j];
+    for (int i = j + threadIdx.x; i < uc; i += blockDim.x) {
+      const int pi = r[i];
+      const int row = pi > pj ? pi : pj, col = pi > pj ? pj : pi;
+      const T v = Uc[i + (long long)j * uc];
+      if (col < sp) L[p.offL + row + (long long)col * mp] += v;
+      else U[p.offU + (row - sp) + (long long)(col - sp) * up] += v;
+    }
+  }
+}
+
+#define TAMD_INST(T)                                                                   \
+  template __global__ void k_fill<T>(T*, long long);                                   \
+  template __global__ void k_assemble<T>(int, const int*, const long long*,             \
+                                         const double*, T*);                            \
+  template __global__ void k_front_small<T>(const FrontDesc*, const int*, T*, T*, int*); \
+  template __global__ void k_potrf_diag<T>(const FrontDesc*, const PanelTask*, T*, int*); \
+  template __global__ void k_trsm_rows<T>(const FrontDesc*, const PanelTask*, T*);      \
+  template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*);           \
+  template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*);
+TAMD_INST(double)
+TAMD_INST(float)
+template __global__ void k_fill<int>(int*, long long);
+
+}  // namespace tamd

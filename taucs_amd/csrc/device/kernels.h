@@ -1,0 +1,56 @@
+// Device kernels of the MI355X multifrontal Cholesky (gfx950, wave64).
+//
+// Storage (all column-major, one arena per kind):
+//   L  panels: supernode f owns m_f x s_f at L + offL, leading dim m_f = s_f+u_f;
+//              rows 0..s-1 are the diagonal block (F1), rows s..m-1 the
+//              sub-diagonal block (F2) of src/taucs_sn_llt.c:1086-1194.
+//   U  update (contribution) blocks: u_f x u_f at U + offU, lower part used.
+//   W  solve workspace: m_f x nrhs per front.
+// Work is described by task arrays built once on the host per level schedule;
+// every kernel reads its task by blockIdx and has no inter-block dependence.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+struct FrontDesc {
+  long long offL;   // panel offset (elements)
+  long long offU;   // update block offset
+  long long offW;   // solve workspace offset (elements per rhs column: m)
+  int s, m;         // columns, rows (m = s + u)
+  int parent;       // index into the front table, -1 at a root
+  int rel;          // offset of this front's update rows in the rel array
+  int rows;         // offset of this front's row list in the rows array
+  int pad;
+};
+
+struct GemmTask {   // C[i,j] -= sum_k A[i,k] * B[j,k] on one 64x64 tile
+  long long a_off;  // A(i,k) = L[a_off + i + k*ld]
+  long long b_off;  // B(j,k) = L[b_off + j + k*ld]
+  long long c_off;  // C(i,j) = C[c_off + i + j*ldc]
+  int ld, ldc;
+  int M, N, K;
+  int flags;        // 1: C lives in U (else in L); 2: diagonal tile, lower part only
+};
+
+struct PanelTask {  // one diagonal block, or one row block of the panel
+  int front;
+  int k0, kb;       // columns k0 .. k0+kb-1 of the panel
+  int r0, nr;       // rows r0 .. r0+nr-1 (trsm only)
+};
+
+struct EATask {     // extend-add columns j0..j1-1 of a child's update block
+  int child;
+  int j0, j1;
+  int pad;
+};
+
+#define TAMD_OP_ASSEMBLE 1
+#define TAMD_OP_SMALL    2
+#define TAMD_OP_POTRF    3
+#define TAMD_OP_TRSM     4
+#define TAMD_OP_GEMM     5
+#define TAMD_OP_EXTEND   6

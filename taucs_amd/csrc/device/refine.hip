@@ -1,0 +1,375 @@
+// FP64 residuals, norms and refinement on the device.
+//
+// r = b - A x is the symmetric SpMV of src/taucs_ccs_ops.c:299-344 (lower
+// storage, mirrored) evaluated on a full-pattern CSR copy so each row is one
+// thread's deterministic sum.  Refinement follows the reference's single
+// precision precedent (progs/direct.c:656-663: FP64-accumulated residual,
+// correction solved with the low-precision factor) and its preconditioned-CG
+// route (src/taucs_linsolve.c:561-567: any factor as the preconditioner of
+// taucs_conjugate_gradients, src/taucs_iter.c:65-222).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "kernels.h"
+#include "../host/internal.h"
+#include "../../../include/taucs_amd.h"
+
+#define HIPOK(x)                                                                     \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) {                                                          \
+      taucs_printf((char*)"taucs_amd: HIP error %s at %s:%d\n", hipGetErrorString(e_), \
+                   __FILE__, __LINE__);                                              \
+      return -1;                                                                     \
+    }                                                                                \
+  } while (0)
+
+namespace tamd {
+
+// y(:,r) = b(:,r) - A x(:,r)     (b may be null: y = A x)
+__global__ void k_csr_residual(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                               const double* __restrict__ v, const double* __restrict__ x,
+                               int ldx, const double* __restrict__ b, int ldb,
+                               double* __restrict__ y, int ldy, int nrhs)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < nrhs; r++) {
+    double acc = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; k++) acc += v[k] * x[ci[k] + (long long)r * ldx];
+    y[i + (long long)r * ldy] = b ? b[i + (long long)r * ldb] - acc : acc;
+  }
+}
+
+// |A| row sums (for |A|_inf)
+__global__ void k_csr_absrow(int n, const int* rp, const double* v, double* out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int k = rp[i]; k < rp[i + 1]; k++) acc += fabs(v[k]);
+  out[i] = acc;
+}
+
+// Per-column partial reductions: mode 0 sum(a*b), 1 max|a|, 2 sum(a*a).
+// partial[(blk) + r*gridDim.x]
+__global__ __launch_bounds__(256) void k_colreduce(int n, const double* __restrict__ a, int lda,
+                                                   const double* __restrict__ b, int ldb,
+                                                   int mode, double* __restrict__ partial)
+{
+  __shared__ double red[256];
+  const int r = blockIdx.y;
+  double acc = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    double av = a[i + (long long)r * lda];
+    if (mode == 0) acc += av * b[i + (long long)r * ldb];
+    else if (mode == 1) acc = fmax(acc, fabs(av));
+    else acc += av * av;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      red[threadIdx.x] = mode == 1 ? fmax(red[threadIdx.x], red[threadIdx.x + off])
+                                   : red[threadIdx.x] + red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x + (long long)r * gridDim.x] = red[0];
+}
+
+__global__ void k_colfinal(int nblk, int mode, const double* partial, double* out, int nrhs)
+{
+  int r = threadIdx.x;
+  if (r >= nrhs) return;
+  double acc = 0.0;
+  for (int q = 0; q < nblk; q++) {
+    double v = partial[q + (long long)r * nblk];
+    acc = mode == 1 ? fmax(acc, v) : acc + v;
+  }
+  out[r] = acc;
+}
+
+// y(:,r) += alpha[r] * x(:,r)   (sign: +1 or -1 applied to alpha)
+__global__ void k_axpy_col(int n, const double* __restrict__ alpha, double sign,
+                           const double* __restrict__ x, double* __restrict__ y, int nrhs)
+{
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * nrhs) return;
+  int r = (int)(e / n);
+  y[e] += sign * alpha[r] * x[e];
+}
+
+// p = z + beta p
+__global__ void k_xpby_col(int n, const double* __restrict__ z, const double* __restrict__ beta,
+                           double* __restrict__ p, int nrhs)
+{
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * nrhs) return;
+  int r = (int)(e / n);
+  p[e] = z[e] + beta[r] * p[e];
+}
+
+// out[r] = (den[r] != 0) ? num[r]/den[r] : 0, optionally masked by done[r]
+__global__ void k_ratio(const double* num, const double* den, const int* done, double* out, int nrhs)
+{
+  int r = threadIdx.x;
+  if (r >= nrhs) return;
+  out[r] = (den[r] != 0.0 && !(done && done[r])) ? num[r] / den[r] : 0.0;
+}
+
+}  // namespace tamd
+
+using namespace tamd;
+
+struct RefineCtx {
+  const taucs_ccs_matrix* A = nullptr;
+  const double* vals_ptr = nullptr;
+  int n = 0, nnz = 0;
+  int* rp = nullptr;
+  int* ci = nullptr;
+  double* v = nullptr;
+  double anorm_inf = 0.0;
+  int cap = 0;  // rhs capacity of the work vectors
+  double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *x = nullptr, *b = nullptr;
+  double *partial = nullptr, *s1 = nullptr, *s2 = nullptr, *s3 = nullptr;
+  int* done = nullptr;
+};
+
+struct CtxEntry { void* key; RefineCtx* ctx; };
+static std::vector<CtxEntry> g_ctx;
+
+static void ctx_free(RefineCtx* c)
+{
+  if (!c) return;
+  void* ps[] = {c->rp, c->ci, c->v, c->r, c->z, c->p, c->q, c->x, c->b, c->partial, c->s1, c->s2, c->s3, c->done};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  delete c;
+}
+
+extern "C" void tamd_refine_forget(void* key)
+{
+  for (size_t i = 0; i < g_ctx.size(); i++)
+    if (g_ctx[i].key == key) {
+      ctx_free(g_ctx[i].ctx);
+      g_ctx.erase(g_ctx.begin() + i);
+      return;
+    }
+}
+
+static int dmalloc(void** p, size_t bytes)
+{
+  if (tamd_alloc_should_fail()) return -1;
+  return hipMalloc(p, bytes ? bytes : 1) == hipSuccess ? 0 : -1;
+}
+
+// Build (or reuse) the full-pattern CSR of A on the device.
+static RefineCtx* get_ctx(void* key, const taucs_ccs_matrix* A, int nrhs)
+{
+  RefineCtx* c = nullptr;
+  for (auto& e : g_ctx)
+    if (e.key == key) c = e.ctx;
+  if (c && (c->A != A || c->vals_ptr != A->values.d || c->n != A->n || c->nnz != A->colptr[A->n])) {
+    tamd_refine_forget(key);
+    c = nullptr;
+  }
+  if (!c) {
+    c = new RefineCtx();
+    int n = A->n, nl = A->colptr[n];
+    std::vector<int> cnt(n + 1, 0);
+    for (int j = 0; j < n; j++)
+      for (int k = A->colptr[j]; k < A->colptr[j + 1]; k++) {
+        int i = A->rowind[k];
+        cnt[i + 1]++;
+        if (i != j) cnt[j + 1]++;
+      }
+    for (int i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+    int nnz = cnt[n];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1), ci(nnz);
+    std::vector<double> v(nnz);
+    for (int j = 0; j < n; j++)
+      for (int k = A->colptr[j]; k < A->colptr[j + 1]; k++) {
+        int i = A->rowind[k];
+        double a = A->values.d[k];
+        ci[fill[i]] = j; v[fill[i]++] = a;
+        if (i != j) { ci[fill[j]] = i; v[fill[j]++] = a; }
+      }
+    double an = 0.0;
+    for (int i = 0; i < n; i++) {
+      double s = 0.0;
+      for (int k = cnt[i]; k < cnt[i + 1]; k++) s += fabs(v[k]);
+      an = std::max(an, s);
+    }
+    c->A = A; c->vals_ptr = A->values.d; c->n = n; c->nnz = nl; c->anorm_inf = an;
+    if (dmalloc((void**)&c->rp, sizeof(int) * (n + 1)) || dmalloc((void**)&c->ci, sizeof(int) * nnz)
+        || dmalloc((void**)&c->v, sizeof(double) * nnz)) {
+      ctx_free(c);
+      return nullptr;
+    }
+    if (hipMemcpy(c->rp, cnt.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice) != hipSuccess
+        || hipMemcpy(c->ci, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice) != hipSuccess
+        || hipMemcpy(c->v, v.data(), sizeof(double) * nnz, hipMemcpyHostToDevice) != hipSuccess) {
+      ctx_free(c);
+      return nullptr;
+    }
+    g_ctx.push_back({key, c});
+  }
+  if (c->cap < nrhs) {
+    void** vs[] = {(void**)&c->r, (void**)&c->z, (void**)&c->p, (void**)&c->q, (void**)&c->x, (void**)&c->b};
+    for (void** pp : vs) {
+      if (*pp) (void)hipFree(*pp);
+      *pp = nullptr;
+      if (dmalloc(pp, sizeof(double) * (size_t)c->n * nrhs)) return nullptr;
+    }
+    void** ss[] = {(void**)&c->partial, (void**)&c->s1, (void**)&c->s2, (void**)&c->s3, (void**)&c->done};
+    size_t sz[] = {sizeof(double) * 256 * (size_t)nrhs, sizeof(double) * nrhs, sizeof(double) * nrhs,
+                   sizeof(double) * nrhs, sizeof(int) * nrhs};
+    for (int i = 0; i < 5; i++) {
+      if (*ss[i]) (void)hipFree(*ss[i]);
+      *ss[i] = nullptr;
+      if (dmalloc(ss[i], sz[i])) return nullptr;
+    }
+    c->cap = nrhs;
+  }
+  return c;
+}
+
+// out[r] = reduce over column r; blocking (returns after the stream syncs if host != null)
+static void colreduce(hipStream_t st, RefineCtx* c, const double* a, const double* b, int mode,
+                      double* dout, int nrhs)
+{
+  int nblk = std::min(256, (c->n + 255) / 256);
+  if (nblk < 1) nblk = 1;
+  hipLaunchKernelGGL(k_colreduce, dim3(nblk, nrhs), dim3(256), 0, st, c->n, a, c->n, b, c->n, mode,
+                     c->partial);
+  hipLaunchKernelGGL(k_colfinal, dim3(1), dim3(64), 0, st, nblk, mode, c->partial, dout, nrhs);
+}
+
+extern "C" int tamd_dev_stream_of(tamd_factor* F, hipStream_t* st);
+extern "C" int tamd_dev_solve_device(tamd_factor* F, int nrhs, double* dX, int ldx, const double* dB, int ldb);
+
+// Backward errors of x: r = b - A x,  eta = |r|_inf / (|A|_inf |x|_inf + |b|_inf)
+static int residual_and_error(hipStream_t st, RefineCtx* c, const double* x, const double* b,
+                              double* r, int nrhs, std::vector<double>& eta,
+                              std::vector<double>& relres)
+{
+  int n = c->n;
+  hipLaunchKernelGGL(k_csr_residual, dim3((n + 255) / 256), dim3(256), 0, st, n, c->rp, c->ci, c->v, x,
+                     n, b, n, r, n, nrhs);
+  std::vector<double> rinf(nrhs), xinf(nrhs), binf(nrhs), r2(nrhs), b2(nrhs);
+  colreduce(st, c, r, nullptr, 1, c->s1, nrhs);
+  HIPOK(hipMemcpyAsync(rinf.data(), c->s1, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+  colreduce(st, c, x, nullptr, 1, c->s2, nrhs);
+  HIPOK(hipMemcpyAsync(xinf.data(), c->s2, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+  colreduce(st, c, b, nullptr, 1, c->s3, nrhs);
+  HIPOK(hipMemcpyAsync(binf.data(), c->s3, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+  HIPOK(hipStreamSynchronize(st));
+  colreduce(st, c, r, nullptr, 2, c->s1, nrhs);
+  HIPOK(hipMemcpyAsync(r2.data(), c->s1, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+  colreduce(st, c, b, nullptr, 2, c->s2, nrhs);
+  HIPOK(hipMemcpyAsync(b2.data(), c->s2, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+  HIPOK(hipStreamSynchronize(st));
+  eta.resize(nrhs);
+  relres.resize(nrhs);
+  for (int q = 0; q < nrhs; q++) {
+    double den = c->anorm_inf * xinf[q] + binf[q];
+    eta[q] = den > 0 ? rinf[q] / den : 0.0;
+    relres[q] = b2[q] > 0 ? sqrt(r2[q]) / sqrt(b2[q]) : sqrt(r2[q]);
+  }
+  return 0;
+}
+
+extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, double* X, int ldx,
+                                       const double* B, int ldb, int method, double tol, int maxit,
+                                       taucs_amd_refine_info* info)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  if (!F || !F->has_numeric || !A || A->n != F->n || nrhs <= 0) return -1;
+  hipStream_t st;
+  if (tamd_dev_stream_of(F, &st) != 0) return -1;
+  RefineCtx* c = get_ctx(vL, A, nrhs);
+  if (!c) return -1;
+  int n = c->n;
+  if (maxit <= 0) maxit = 100;
+  HIPOK(hipMemcpy2DAsync(c->b, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                         hipMemcpyHostToDevice, st));
+  std::vector<double> eta, relres;
+  int it = 0;
+  bool conv = false;
+  const unsigned gv = (unsigned)(((long long)n * nrhs + 255) / 256);
+  if (method == 0) {
+    // x = M^-1 b; repeat r = b - A x, x += M^-1 r
+    if (tamd_dev_solve_device(F, nrhs, c->x, n, c->b, n)) return -1;
+    for (;;) {
+      if (residual_and_error(st, c, c->x, c->b, c->r, nrhs, eta, relres)) return -1;
+      double worst = *std::max_element(eta.begin(), eta.end());
+      if (worst <= tol) { conv = true; break; }
+      if (it >= maxit) break;
+      if (tamd_dev_solve_device(F, nrhs, c->z, n, c->r, n)) return -1;
+      std::vector<double> ones(nrhs, 1.0);
+      HIPOK(hipMemcpyAsync(c->s3, ones.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_axpy_col, dim3(gv), dim3(256), 0, st, n, c->s3, 1.0, c->z, c->x, nrhs);
+      it++;
+    }
+  } else {
+    // preconditioned CG, one independent recurrence per right-hand side
+    HIPOK(hipMemsetAsync(c->x, 0, sizeof(double) * n * nrhs, st));
+    HIPOK(hipMemcpyAsync(c->r, c->b, sizeof(double) * n * nrhs, hipMemcpyDeviceToDevice, st));
+    if (tamd_dev_solve_device(F, nrhs, c->z, n, c->r, n)) return -1;
+    HIPOK(hipMemcpyAsync(c->p, c->z, sizeof(double) * n * nrhs, hipMemcpyDeviceToDevice, st));
+    HIPOK(hipMemsetAsync(c->done, 0, sizeof(int) * nrhs, st));
+    double* rz = nullptr;
+    double* rz_new = nullptr;
+    double* scal = nullptr;
+    if (dmalloc((void**)&rz, sizeof(double) * nrhs) || dmalloc((void**)&rz_new, sizeof(double) * nrhs)
+        || dmalloc((void**)&scal, sizeof(double) * nrhs)) {
+      if (rz) (void)hipFree(rz);
+      if (rz_new) (void)hipFree(rz_new);
+      return -1;
+    }
+    std::vector<int> done(nrhs, 0);
+    colreduce(st, c, c->r, c->z, 0, rz, nrhs);
+    for (;;) {
+      // q = A p ; alpha = rz / (p.q)
+      hipLaunchKernelGGL(k_csr_residual, dim3((n + 255) / 256), dim3(256), 0, st, n, c->rp, c->ci, c->v,
+                         c->p, n, (const double*)nullptr, n, c->q, n, nrhs);
+      colreduce(st, c, c->p, c->q, 0, c->s1, nrhs);
+      hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, st, rz, c->s1, c->done, scal, nrhs);
+      hipLaunchKernelGGL(k_axpy_col, dim3(gv), dim3(256), 0, st, n, scal, 1.0, c->p, c->x, nrhs);
+      hipLaunchKernelGGL(k_axpy_col, dim3(gv), dim3(256), 0, st, n, scal, -1.0, c->q, c->r, nrhs);
+      it++;
+      // convergence on the true residual every iteration (k small)
+      if (residual_and_error(st, c, c->x, c->b, c->q, nrhs, eta, relres)) break;
+      bool all = true;
+      for (int q = 0; q < nrhs; q++) {
+        if (eta[q] <= tol) done[q] = 1;
+        all = all && done[q];
+      }
+      if (all) { conv = true; break; }
+      if (it >= maxit) break;
+      HIPOK(hipMemcpyAsync(c->done, done.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+      if (tamd_dev_solve_device(F, nrhs, c->z, n, c->r, n)) break;
+      colreduce(st, c, c->r, c->z, 0, rz_new, nrhs);
+      hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, st, rz_new, rz, (const int*)nullptr, scal, nrhs);
+      hipLaunchKernelGGL(k_xpby_col, dim3(gv), dim3(256), 0, st, n, c->z, scal, c->p, nrhs);
+      HIPOK(hipMemcpyAsync(rz, rz_new, sizeof(double) * nrhs, hipMemcpyDeviceToDevice, st));
+    }
+    (void)hipFree(rz);
+    (void)hipFree(rz_new);
+    (void)hipFree(scal);
+    residual_and_error(st, c, c->x, c->b, c->r, nrhs, eta, relres);
+  }
+  HIPOK(hipMemcpy2DAsync(X, sizeof(double) * ldx, c->x, sizeof(double) * n, sizeof(double) * n, nrhs,
+                         hipMemcpyDeviceToHost, st));
+  HIPOK(hipStreamSynchronize(st));
+  if (info) {
+    info->iterations = it;
+    info->converged = conv;
+    info->backward_error = eta.empty() ? 0 : *std::max_element(eta.begin(), eta.end());
+    info->rel_residual = relres.empty() ? 0 : *std::max_element(relres.begin(), relres.end());
+  }
+  return 0;
+}
