@@ -1,0 +1,208 @@
+"""Benchmark of the MI355X supernodal multifrontal Cholesky (one GPU).
+
+Workload (default): the plain FP64 GPU path on the 3D 7-point Laplacian of a
+60x60x60 grid (n = 216,000), genmmd ordering, 8 right-hand sides uniform[-1,1]
+(seeds 1..8).  One step = numeric factorization of P A P^T (symbolic analysis
+done once, outside the timed region, as the reference's symbolic/numeric split
+allows) + the forward/backward supernodal solve of the 8 right-hand sides, with
+matrix values and right-hand sides resident in HBM... except A's values, which
+the TAUCS numeric entry point takes from host memory (7 MB, inside the step).
+
+Printed JSON line: value = factorization GFLOP/s (reference flop count: the
+fundamental-supernode count of src/taucs_sn_llt.c:3911-3921, over the mean
+numeric time); also solves/s, p50/p99 step latency, the mixed-precision
+(FP32 fronts + FP64 refinement) figures on the same workload, the roofline of
+the dominant kernel (the FP64 MFMA tile GEMM/SYRK) from device events, and the
+CPU baseline (the oracle's LAPACK-per-front multifrontal factorization on a
+bounded sample, timed on the host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12   # MI355X dense FP64 MFMA
+PEAK_FP32 = 157.3e12  # MI355X dense FP32 MFMA
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sample_side=30, nrhs=8):
+    """Oracle multifrontal factor (scipy LAPACK per front, as the reference
+    calls BLAS/LAPACK per front) on mesh3d(sample_side), timed on the host."""
+    from oracle import oracle as O
+    n, cp, ri, v = O.mesh3d(sample_side, sample_side, sample_side)
+    perm, inv = O.genmmd(n, cp, ri) if O.have_ref() else (None, None)
+    if perm is None:
+        import taucs_amd as T
+        perm, inv = T.order(T.mesh3d(sample_side), "genmmd")
+    pcp, pri, pv = O.permute(n, cp, ri, v, inv)
+    S = O.symbolic(n, pcp, pri)
+    t0 = time.perf_counter()
+    panels = O.factor(S, pcp, pri, pv)
+    tf = time.perf_counter() - t0
+    b = np.random.default_rng(0).uniform(-1, 1, (n, nrhs))
+    t0 = time.perf_counter()
+    O.solve(S, panels, b)
+    ts = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(S["fund_flops"] / tf / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle multifrontal LL^T (scipy LAPACK per front) of mesh3d({sample_side}^3), "
+                      f"genmmd, fundamental flops {S['fund_flops']:.3e}, factor {tf:.2f} s, "
+                      f"{nrhs}-rhs solve {ts:.2f} s ({nrhs / ts:.2f} solves/s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--side", type=int, default=60)
+    ap.add_argument("--nrhs", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU layout is a design in DESIGN.md, not part of this build)")
+        sys.exit(2)
+
+    import taucs_amd as T
+    side, k = args.side, args.nrhs
+    log(f"bench: mesh3d {side}^3, {k} rhs, steps {args.steps} warmup {args.warmup}")
+    A = T.mesh3d(side)
+    n = A.n
+    perm, inv = T.order(A, "genmmd")
+    PA = T.permute(A, perm, inv)
+    F = T.factor_symbolic(PA)
+    fi = F.info()
+    B = np.stack([T.random_uniform(n, 1 + r) for r in range(k)], 1)
+    dB = T.DeviceArray(B)
+    dX = T.DeviceArray(np.zeros((n, k)))
+
+    def step(factor_obj, prec):
+        t0 = time.perf_counter()
+        if factor_obj.numeric(PA, prec) != 0:
+            raise RuntimeError("numeric factorization failed")
+        t1 = time.perf_counter()
+        factor_obj.solve_device(dX.ptr, dB.ptr, n, k)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1
+
+    for _ in range(args.warmup):
+        step(F, T.FP64)
+    T.lib.taucs_amd_device_synchronize()
+    tf, ts = [], []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        a, b = step(F, T.FP64)
+        tf.append(a)
+        ts.append(b)
+    T.lib.taucs_amd_device_synchronize()
+    t_total = time.perf_counter() - t_start
+    X = dX.get()
+    M = PA.to_scipy_full()
+    R = B - M @ X
+    relres = float((np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)).max())
+    anorm = abs(M).sum(1).max()
+    bwd = float((abs(R).max(0) / (anorm * abs(X).max(0) + abs(B).max(0))).max())
+    tf_mean, ts_mean = float(np.mean(tf)), float(np.mean(ts))
+    step_ms = 1e3 * t_total / args.steps
+    lat = 1e3 * (np.array(tf) + np.array(ts))
+    log(f"fp64: factor {tf_mean*1e3:.2f} ms ({fi.fund_flops/tf_mean/1e9:.1f} GFLOP/s), solve {ts_mean*1e3:.2f} ms, "
+        f"relres {relres:.2e} bwd {bwd:.2e}")
+
+    # one refinement step brings the FP64 residual to working precision
+    x_ref, info_ref = F.solve_refined(PA, B, method=0, tol=1e-15, maxit=1)
+
+    # roofline of the dominant kernel, from device events on the factor's stream
+    T.lib.taucs_amd_set_profiling(1)
+    step(F, T.FP64)
+    T.lib.taucs_amd_set_profiling(0)
+    tm = F.timing()
+    gemm_avg_ms = tm.gemm_ms / max(tm.gemm_launches, 1)
+    gemm_rate = tm.gemm_flops / (tm.gemm_ms * 1e-3) if tm.gemm_ms > 0 else 0.0
+    log(f"profile: total {tm.total_ms:.2f} ms, front {tm.front_ms:.2f} ms, gemm {tm.gemm_ms:.2f} ms "
+        f"({tm.gemm_launches} launches, {gemm_rate/1e12:.2f} TFLOP/s), assembly {tm.assembly_ms:.2f} ms, "
+        f"launches {tm.launches}")
+
+    mixed = None
+    if not args.no_mixed:
+        Fm = T.factor_symbolic(PA)
+        tmf, tms, its, bw = [], [], [], []
+        for i in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            if Fm.numeric(PA, T.FP32) != 0:
+                raise RuntimeError("fp32 factorization failed")
+            t1 = time.perf_counter()
+            xm, info = Fm.solve_refined(PA, B, method=0, tol=1e-12, maxit=50)
+            t2 = time.perf_counter()
+            if i >= args.warmup:
+                tmf.append(t1 - t0)
+                tms.append(t2 - t1)
+                its.append(info.iterations)
+                bw.append(info.backward_error)
+        mixed = {"precision": "fp32 fronts + fp64 refinement",
+                 "factor_gflops": round(fi.fund_flops / np.mean(tmf) / 1e9, 2),
+                 "factor_ms": round(1e3 * float(np.mean(tmf)), 3),
+                 "refined_solve_ms": round(1e3 * float(np.mean(tms)), 3),
+                 "refine_iterations": int(max(its)),
+                 "backward_error": float(max(bw)),
+                 "speedup_factor_plus_solve_vs_fp64": round((tf_mean + ts_mean) / (np.mean(tmf) + np.mean(tms)), 3)}
+        log(f"mixed: {mixed}")
+        Fm.free()
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(30 if side >= 30 else side, k)
+        except Exception as e:  # the baseline must not hide the GPU result
+            cpu = {"error": str(e)}
+
+    out = {
+        "metric": "factor GFLOP/s & solves/s, mixed vs FP64 at bkwd err <=1e-12; p50/p99 latency",
+        "value": round(fi.fund_flops / tf_mean / 1e9, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"mesh3d {side}x{side}x{side} 7-point Laplacian, genmmd ordering, "
+                               f"numeric factor + {k}-rhs supernodal solve (plain FP64 path)",
+                   "n": n, "nrhs": k, "fund_flops": fi.fund_flops, "relax_flops": fi.relax_flops,
+                   "n_supernodes": fi.n_live, "levels": fi.n_levels, "max_front": fi.max_front},
+        "factor_ms": round(1e3 * tf_mean, 3),
+        "solve_ms": round(1e3 * ts_mean, 3),
+        "solves_per_s": round(k / ts_mean, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3)},
+        "rel_residual": relres,
+        "backward_error": bwd,
+        "rel_residual_after_1_refinement": info_ref.rel_residual,
+        "mixed": mixed,
+        "roofline": {"bound": "mfma", "kernel": "k_gemm_tile<double> (v_mfma_f64_16x16x4)",
+                     "achieved": round(gemm_rate / 1e12, 3), "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": round(gemm_rate / PEAK_FP64, 4),
+                     "avg_launch_ms": round(gemm_avg_ms, 4), "launches": tm.gemm_launches,
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -86,6 +86,13 @@ int taucs_amd_solve_refined(void* L, taucs_ccs_matrix* A, int nrhs, double* X, i
                             const double* B, int ldb, int method, double tol, int maxit,
                             taucs_amd_refine_info* info);
 
+/* ---- device buffers (for right-hand sides kept in HBM) ------------------- */
+void* taucs_amd_device_malloc(size_t bytes);
+void  taucs_amd_device_free(void* p);
+int   taucs_amd_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int   taucs_amd_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int   taucs_amd_device_synchronize(void);
+
 /* ---- timing of the last numeric factorization (device events) ----------- */
 typedef struct {
   double total_ms;       /* numeric factorization, stream time */
@@ -93,7 +100,7 @@ typedef struct {
   double assembly_ms;    /* A scatter + extend-add */
   double gemm_ms;        /* tile GEMM/SYRK kernels only */
   int    gemm_launches;
-  double gemm_flops;     /* executed tile-GEMM flops */
+  double gemm_flops;     /* algorithmic tile-GEMM/SYRK flops (lower part of diagonal tiles) */
   double assembly_bytes; /* algorithmic bytes of A scatter + extend-add */
   int    launches;
 } taucs_amd_numeric_timing;

@@ -94,6 +94,11 @@ _sigs = {
                                           C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int,
                                           P(RefineInfo)]),
     "taucs_amd_set_profiling": (None, [C.c_int]),
+    "taucs_amd_device_malloc": (C.c_void_p, [C.c_size_t]),
+    "taucs_amd_device_free": (None, [C.c_void_p]),
+    "taucs_amd_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "taucs_amd_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "taucs_amd_device_synchronize": (C.c_int, []),
     "taucs_amd_get_numeric_timing": (C.c_int, [C.c_void_p, P(NumericTiming)]),
 }
 for _name, (_res, _args) in _sigs.items():
@@ -280,6 +285,12 @@ class Factor:
             raise RuntimeError("refined solve failed")
         return (X[:, 0].copy() if one else X), info
 
+    def solve_device(self, dX, dB, n, nrhs):
+        """Solve with right-hand sides and solutions already in HBM."""
+        rc = lib.taucs_amd_supernodal_solve_many_device(self.h, nrhs, dX, n, dB, n)
+        if rc != 0:
+            raise RuntimeError("device solve failed")
+
     def to_ccs(self):
         return Matrix(lib.taucs_supernodal_factor_to_ccs(self.h))
 
@@ -349,3 +360,32 @@ def gpu_available():
         return "gfx950" in r.stdout
     except Exception:
         return False
+
+
+class DeviceArray:
+    """A column-major float64 array in HBM (n x k)."""
+
+    def __init__(self, host):
+        host = np.asfortranarray(np.asarray(host, dtype=np.float64))
+        self.shape = host.shape
+        self.nbytes = host.nbytes
+        self.ptr = lib.taucs_amd_device_malloc(max(self.nbytes, 8))
+        if not self.ptr:
+            raise MemoryError("device allocation failed")
+        lib.taucs_amd_memcpy_h2d(self.ptr, host.ctypes.data, self.nbytes)
+
+    def get(self):
+        out = np.empty(self.shape, dtype=np.float64, order="F")
+        lib.taucs_amd_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes)
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib.taucs_amd_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -156,6 +156,13 @@ extern "C" void tamd_dev_free(tamd_factor* F)
 
 // ---- schedule construction --------------------------------------------------
 
+// Algorithmic flops of a tile: a diagonal tile only needs its lower part.
+static double tile_flops(const GemmTask& g)
+{
+  double entries = (g.flags & 2) ? (double)g.N * g.M - 0.5 * g.N * (g.N - 1) : (double)g.M * g.N;
+  return 2.0 * g.K * entries;
+}
+
 static const int SMALL_M = 64;   // fronts with m <= 64 run whole in one wave
 static const int NB = 64;        // panel block width of large fronts
 
@@ -241,7 +248,7 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
             g.K = kb;
             g.flags = (r0 == c0) ? 2 : 0;
             gemm.push_back(g);
-            D->gemm_flops += 2.0 * g.M * g.N * g.K;
+            D->gemm_flops += tile_flops(g);
           }
       }
       if ((int)potrf.size() > p0) D->ops.push_back({TAMD_OP_POTRF, p0, (int)potrf.size() - p0, l});
@@ -268,7 +275,7 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
             g.K = d.s;
             g.flags = 1 | (i0 == j0 ? 2 : 0);
             gemm.push_back(g);
-            D->gemm_flops += 2.0 * g.M * g.N * g.K;
+            D->gemm_flops += tile_flops(g);
           }
       }
       if ((int)gemm.size() > g0) D->ops.push_back({TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l});
@@ -595,4 +602,25 @@ extern "C" int tamd_dev_download_blocks(tamd_factor* F, double* Lhost)
     for (size_t i = 0; i < n; i++) Lhost[i] = tmp[i];
   }
   return 0;
+}
+
+// ---- device buffers for callers that keep right-hand sides in HBM ----------
+extern "C" void* taucs_amd_device_malloc(size_t bytes)
+{
+  void* p = nullptr;
+  if (dev_malloc(&p, bytes) != 0) return nullptr;
+  return p;
+}
+extern "C" void taucs_amd_device_free(void* p) { free_ptr(p); }
+extern "C" int taucs_amd_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+}
+extern "C" int taucs_amd_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+extern "C" int taucs_amd_device_synchronize(void)
+{
+  return hipDeviceSynchronize() == hipSuccess ? 0 : -1;
 }
