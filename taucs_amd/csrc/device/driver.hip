@@ -15,16 +15,20 @@
 namespace tamd {
 template <typename T> __global__ void k_fill(T*, long long);
 template <typename T> __global__ void k_assemble(int, const int*, const long long*, const double*, T*);
-template <typename T> __global__ void k_front_small(const FrontDesc*, const int*, T*, T*, int*);
-template <typename T> __global__ void k_potrf_diag(const FrontDesc*, const PanelTask*, T*, int*);
-template <typename T> __global__ void k_trsm_rows(const FrontDesc*, const PanelTask*, T*);
-template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*);
+template <typename T> __global__ void k_front_small(const FrontDesc*, const int*, T*, T*, T*, int*);
+template <typename T> __global__ void k_potrf_diag(const FrontDesc*, const PanelTask*, T*, T*, int*);
+template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*, const T*);
 __global__ void k_gemm_tile_bf16(const GemmTask*, const float*, float*, float*);
 template <typename T> __global__ void k_extend_add(const FrontDesc*, const EATask*, const int*, T*, T*);
 template <typename T> __global__ void k_solve_init(const FrontDesc*, const int*, const double*, int, T*, int);
-template <typename T> __global__ void k_solve_fwd(const FrontDesc*, const int*, const T*, T*, int);
-template <typename T> __global__ void k_solve_ext(const FrontDesc*, const int*, const int*, T*, int);
-template <typename T> __global__ void k_solve_bwd(const FrontDesc*, const int*, const int*, const T*, T*, double*, int, int);
+template <typename T> __global__ void k_fwd_step(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, T*, T*, int, int);
+template <typename T> __global__ void k_fwd_rect(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, int, T*, int);
+template <typename T> __global__ void k_fwd_rect_reduce(const FrontDesc*, const PanelTask*, const T*, T*, int);
+template <typename T> __global__ void k_bwd_cols_reduce(const FrontDesc*, const PanelTask*, const int*, const T*, T*, int, int);
+template <typename T> __global__ void k_solve_ext(const FrontDesc*, const PanelTask*, const int*, T*, int);
+template <typename T> __global__ void k_bwd_gather(const FrontDesc*, const PanelTask*, const int*, const double*, int, T*, int);
+template <typename T> __global__ void k_bwd_cols(const FrontDesc*, const PanelTask*, const T*, const T*, T*, int);
+template <typename T> __global__ void k_bwd_step(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, T*, int, double*, int, int);
 }  // namespace tamd
 
 using namespace tamd;
@@ -48,6 +52,9 @@ struct Op {
   int level;
 };
 
+// solve op kinds
+enum { SV_INIT, SV_FSTEP, SV_FRECT, SV_FRED, SV_EXT, SV_GATHER, SV_BCOLS, SV_BRED, SV_BSTEP };
+
 struct tamd_dev {
   hipStream_t stream = nullptr;
   int elem = 8;  // bytes per factor element
@@ -59,16 +66,14 @@ struct tamd_dev {
   long long* d_adest = nullptr;
   int* d_small = nullptr;        // front indices for k_front_small
   PanelTask* d_potrf = nullptr;
-  PanelTask* d_trsm = nullptr;
+  GemmTask* d_trsm = nullptr;   // panel solves as GEMMs with the block inverses
   GemmTask* d_gemm = nullptr;
   EATask* d_ea = nullptr;
-  int* d_solve_list = nullptr;   // fronts by level (solve)
-  int* d_solve_ext = nullptr;    // children by (level, rank) for the solve
+  PanelTask* d_sv_tasks = nullptr;  // solve tasks
+  std::vector<Op> sv_ops;
   int* d_info = nullptr;
   std::vector<Op> ops;
   std::vector<int> lvl_ptr;      // fronts of level l: [lvl_ptr[l], lvl_ptr[l+1])
-  std::vector<int> sext_ptr;     // solve-ext groups
-  std::vector<int> sext_lvl;
   int n_fronts = 0, ncol = 0, nnz = 0;
   long long W_per_rhs = 0;
   double gemm_flops = 0, assembly_bytes = 0;
@@ -77,10 +82,15 @@ struct tamd_dev {
   int prec = -1;
   void* L = nullptr;
   void* U = nullptr;
+  void* Inv = nullptr;           // inverses of the 64x64 diagonal blocks
+  long long I_elems = 0;
   double* d_avals = nullptr;
   long long L_elems = 0, U_elems = 0;
   // solve buffers
   void* W = nullptr;
+  void* Y = nullptr;             // forward solution, n x W_rhs, by global column
+  void* Part = nullptr;          // split-K partial blocks of the solve (64 x 16 each)
+  long long part_slots = 0;
   int W_rhs = 0;
   double* d_x = nullptr;
   double* d_b = nullptr;
@@ -131,8 +141,11 @@ extern "C" void tamd_dev_free_numeric(tamd_factor* F)
   if (D->stream) (void)hipStreamSynchronize(D->stream);
   free_ptr(D->L);
   free_ptr(D->U);
+  free_ptr(D->Inv);
   free_ptr(D->W);
-  D->L = D->U = D->W = nullptr;
+  free_ptr(D->Y);
+  free_ptr(D->Part);
+  D->L = D->U = D->Inv = D->W = D->Y = D->Part = nullptr;
   D->W_rhs = 0;
   D->prec = -1;
 }
@@ -147,7 +160,7 @@ extern "C" void tamd_dev_free(tamd_factor* F)
   tamd_dev_free_numeric(F);
   free_ptr(D->d_fronts); free_ptr(D->d_rows); free_ptr(D->d_rel); free_ptr(D->d_colptr);
   free_ptr(D->d_adest); free_ptr(D->d_small); free_ptr(D->d_potrf); free_ptr(D->d_trsm);
-  free_ptr(D->d_gemm); free_ptr(D->d_ea); free_ptr(D->d_solve_list); free_ptr(D->d_solve_ext);
+  free_ptr(D->d_gemm); free_ptr(D->d_ea); free_ptr(D->d_sv_tasks);
   free_ptr(D->d_info); free_ptr(D->d_avals); free_ptr(D->d_x); free_ptr(D->d_b);
   if (D->stream) (void)hipStreamDestroy(D->stream);
   delete D;
@@ -187,7 +200,7 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
 
   std::vector<FrontDesc> fr(nf);
   std::vector<int> rows;
-  long long offW = 0;
+  long long offW = 0, offI = 0;
   for (int k = 0; k < nf; k++) {
     int sn = P->order[k];
     FrontDesc d;
@@ -200,15 +213,19 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
     d.parent = P->sn_parent[sn] < P->n_sn ? idx_of[P->sn_parent[sn]] : -1;
     d.rel = P->rel_ptr[sn];
     d.rows = (int)rows.size();
+    d.offI = offI;
     rows.insert(rows.end(), S->sn_struct[sn], S->sn_struct[sn] + d.m);
     offW += d.m;
+    offI += (long long)((d.s + 63) / 64) * 4096;
     fr[k] = d;
   }
   D->W_per_rhs = offW;
+  D->I_elems = offI;
 
   // factorization task lists, level by level
   std::vector<int> small;
-  std::vector<PanelTask> potrf, trsm;
+  std::vector<PanelTask> potrf;
+  std::vector<GemmTask> trsm;
   std::vector<GemmTask> gemm;
   std::vector<EATask> ea;
   D->lvl_ptr.assign(P->level_ptr, P->level_ptr + P->n_levels + 1);
@@ -234,7 +251,19 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
         if (k0 >= d.s) continue;
         int kb = std::min(NB, d.s - k0);
         potrf.push_back({k, k0, kb, 0, 0});
-        for (int r0 = k0 + kb; r0 < d.m; r0 += 64) trsm.push_back({k, k0, kb, r0, std::min(64, d.m - r0)});
+        for (int r0 = k0 + kb; r0 < d.m; r0 += 64) {
+          GemmTask g;
+          g.a_off = d.offL + r0 + (long long)k0 * d.m;
+          g.b_off = d.offI + (long long)(k0 / 64) * 4096;
+          g.c_off = g.a_off;
+          g.ld = d.m;
+          g.ldc = d.m;
+          g.M = std::min(64, d.m - r0);
+          g.N = kb;
+          g.K = kb;
+          g.flags = 4;
+          trsm.push_back(g);
+        }
         for (int c0 = k0 + kb; c0 < d.s; c0 += 64)
           for (int r0 = c0; r0 < d.m; r0 += 64) {
             GemmTask g;
@@ -307,25 +336,101 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
     }
   }
 
-  // solve lists: fronts by level, and children by (level, rank)
-  std::vector<int> slist(nf);
-  for (int k = 0; k < nf; k++) slist[k] = k;
-  std::vector<int> sext;
-  D->sext_ptr.clear();
-  D->sext_lvl.clear();
+  // solve schedule: forward bottom-up, backward top-down, 64-column blocks;
+  // every op is a list of PanelTasks (front, k0, kb, r0, nr)
+  std::vector<PanelTask> svt;
+  D->sv_ops.clear();
+  D->sv_ops.push_back({SV_INIT, 0, nf, -1});
+  auto push_tasks = [&](int kind, size_t t0, int l) {
+    if (svt.size() > t0) D->sv_ops.push_back({kind, (int)t0, (int)(svt.size() - t0), l});
+  };
+  const int SB = 64, CH = 512;
+  D->part_slots = 0;
   for (int l = 0; l < P->n_levels; l++) {
     int b0 = P->level_ptr[l], b1 = P->level_ptr[l + 1];
+    int maxb = 0;
+    for (int k = b0; k < b1; k++) maxb = std::max(maxb, (fr[k].s + SB - 1) / SB);
+    for (int step = 0; step < maxb; step++) {
+      size_t t0 = svt.size();
+      for (int k = b0; k < b1; k++) {
+        int k0 = step * SB;
+        if (k0 >= fr[k].s) continue;
+        int kb = std::min(SB, fr[k].s - k0);
+        if (k0 + kb >= fr[k].s) svt.push_back({k, k0, kb, k0 + kb, 0});  // stores y only
+        for (int r0 = k0 + kb; r0 < fr[k].s; r0 += 64)
+          svt.push_back({k, k0, kb, r0, std::min(64, fr[k].s - r0)});
+      }
+      push_tasks(SV_FSTEP, t0, l);
+    }
+    {  // L21 rows: split over column chunks of CH, then a deterministic reduce
+      size_t t0 = svt.size();
+      std::vector<PanelTask> red;
+      for (int k = b0; k < b1; k++)
+        for (int r0 = fr[k].s; r0 < fr[k].m; r0 += 64) {
+          int nr = std::min(64, fr[k].m - r0), slot0 = (int)(svt.size() - t0), cnt = 0;
+          for (int c0 = 0; c0 < fr[k].s; c0 += CH, cnt++)
+            svt.push_back({k, c0, std::min(CH, fr[k].s - c0), r0, nr});
+          red.push_back({k, slot0, cnt, r0, nr});
+        }
+      D->part_slots = std::max(D->part_slots, (long long)(svt.size() - t0));
+      push_tasks(SV_FRECT, t0, l);
+      t0 = svt.size();
+      svt.insert(svt.end(), red.begin(), red.end());
+      push_tasks(SV_FRED, t0, l);
+    }
     int maxrank = -1;
     for (int k = b0; k < b1; k++)
       if (fr[k].parent >= 0 && fr[k].m > fr[k].s) maxrank = std::max(maxrank, P->sib_rank[P->order[k]]);
     for (int rk = 0; rk <= maxrank; rk++) {
-      D->sext_ptr.push_back((int)sext.size());
-      D->sext_lvl.push_back(l);
-      for (int k = b0; k < b1; k++)
-        if (fr[k].parent >= 0 && fr[k].m > fr[k].s && P->sib_rank[P->order[k]] == rk) sext.push_back(k);
+      size_t t0 = svt.size();
+      for (int k = b0; k < b1; k++) {
+        int u = fr[k].m - fr[k].s;
+        if (fr[k].parent < 0 || u == 0 || P->sib_rank[P->order[k]] != rk) continue;
+        for (int r0 = 0; r0 < u; r0 += 256) svt.push_back({k, 0, 0, r0, std::min(256, u - r0)});
+      }
+      push_tasks(SV_EXT, t0, l);
     }
   }
-  D->sext_ptr.push_back((int)sext.size());
+  for (int l = P->n_levels - 1; l >= 0; l--) {
+    int b0 = P->level_ptr[l], b1 = P->level_ptr[l + 1];
+    size_t t0 = svt.size();
+    for (int k = b0; k < b1; k++) {
+      int u = fr[k].m - fr[k].s;
+      for (int r0 = 0; r0 < u; r0 += 256) svt.push_back({k, 0, 0, r0, std::min(256, u - r0)});
+    }
+    push_tasks(SV_GATHER, t0, l);
+    {  // z -= L21^T x2: 64 columns per task, split over row chunks of CH
+      t0 = svt.size();
+      std::vector<PanelTask> red;
+      for (int k = b0; k < b1; k++) {
+        if (fr[k].m == fr[k].s) continue;
+        for (int c0 = 0; c0 < fr[k].s; c0 += 64) {
+          int nc = std::min(64, fr[k].s - c0), slot0 = (int)(svt.size() - t0), cnt = 0;
+          for (int i0 = fr[k].s; i0 < fr[k].m; i0 += CH, cnt++)
+            svt.push_back({k, i0, std::min(CH, fr[k].m - i0), c0, nc});
+          red.push_back({k, slot0, cnt, c0, nc});
+        }
+      }
+      D->part_slots = std::max(D->part_slots, (long long)(svt.size() - t0));
+      push_tasks(SV_BCOLS, t0, l);
+      t0 = svt.size();
+      svt.insert(svt.end(), red.begin(), red.end());
+      push_tasks(SV_BRED, t0, l);
+    }
+    int maxb = 0;
+    for (int k = b0; k < b1; k++) maxb = std::max(maxb, (fr[k].s + SB - 1) / SB);
+    for (int q = 0; q < maxb; q++) {  // q-th block from the end of each front
+      t0 = svt.size();
+      for (int k = b0; k < b1; k++) {
+        int nbk = (fr[k].s + SB - 1) / SB, blk = nbk - 1 - q;
+        if (blk < 0) continue;
+        int k0 = blk * SB, kb = std::min(SB, fr[k].s - k0);
+        if (k0 == 0) svt.push_back({k, 0, kb, 0, 0});  // stores x only
+        for (int c0 = 0; c0 < k0; c0 += 64) svt.push_back({k, k0, kb, c0, std::min(64, k0 - c0)});
+      }
+      push_tasks(SV_BSTEP, t0, l);
+    }
+  }
 
   D->ncol = S->n;
   D->nnz = P->a_nnz;
@@ -335,8 +440,7 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
       || upload(&D->d_rel, rel, D->stream) || upload(&D->d_adest, adest, D->stream)
       || upload(&D->d_small, small, D->stream) || upload(&D->d_potrf, potrf, D->stream)
       || upload(&D->d_trsm, trsm, D->stream) || upload(&D->d_gemm, gemm, D->stream)
-      || upload(&D->d_ea, ea, D->stream) || upload(&D->d_solve_list, slist, D->stream)
-      || upload(&D->d_solve_ext, sext, D->stream)
+      || upload(&D->d_ea, ea, D->stream) || upload(&D->d_sv_tasks, svt, D->stream)
       || dev_malloc((void**)&D->d_info, sizeof(int))
       || dev_malloc((void**)&D->d_colptr, sizeof(int) * (S->n + 1))
       || dev_malloc((void**)&D->d_avals, sizeof(double) * (P->a_nnz > 0 ? P->a_nnz : 1))) {
@@ -367,6 +471,7 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
   hipStream_t st = D->stream;
   T* L = (T*)D->L;
   T* U = (T*)D->U;
+  T* Iv = (T*)D->Inv;
   hipEvent_t e0 = nullptr, e1 = nullptr, a0 = nullptr, a1 = nullptr;
   const bool prof = g_profiling != 0;
   memset(&D->timing, 0, sizeof(D->timing));
@@ -390,16 +495,16 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
                            D->d_colptr, D->d_adest, D->d_avals, L);
         break;
       case TAMD_OP_SMALL:
-        hipLaunchKernelGGL(k_front_small<T>, dim3(op.count), dim3(64), 0, st, D->d_fronts,
-                           D->d_small + op.start, L, U, D->d_info);
+        hipLaunchKernelGGL(k_front_small<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts,
+                           D->d_small + op.start, L, U, Iv, D->d_info);
         break;
       case TAMD_OP_POTRF:
-        hipLaunchKernelGGL(k_potrf_diag<T>, dim3(op.count), dim3(64), 0, st, D->d_fronts,
-                           D->d_potrf + op.start, L, D->d_info);
+        hipLaunchKernelGGL(k_potrf_diag<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts,
+                           D->d_potrf + op.start, L, Iv, D->d_info);
         break;
       case TAMD_OP_TRSM:
-        hipLaunchKernelGGL(k_trsm_rows<T>, dim3(op.count), dim3(64), 0, st, D->d_fronts,
-                           D->d_trsm + op.start, L);
+        hipLaunchKernelGGL(k_gemm_tile<T>, dim3(op.count), dim3(256), 0, st,
+                           D->d_trsm + op.start, (const T*)L, L, U, (const T*)Iv);
         break;
       case TAMD_OP_GEMM:
         if (bf16upd)
@@ -407,7 +512,7 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
                              D->d_gemm + op.start, (const float*)L, (float*)L, (float*)U);
         else
           hipLaunchKernelGGL(k_gemm_tile<T>, dim3(op.count), dim3(256), 0, st,
-                             D->d_gemm + op.start, (const T*)L, L, U);
+                             D->d_gemm + op.start, (const T*)L, L, U, (const T*)Iv);
         break;
       case TAMD_OP_EXTEND:
         hipLaunchKernelGGL(k_extend_add<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts,
@@ -460,7 +565,8 @@ extern "C" int tamd_dev_numeric(tamd_factor* F, const taucs_ccs_matrix* A)
   if (D->prec != prec) {
     tamd_dev_free_numeric(F);
     if (dev_malloc(&D->L, (size_t)elem * (D->L_elems > 0 ? D->L_elems : 1)) != 0
-        || dev_malloc(&D->U, (size_t)elem * (D->U_elems > 0 ? D->U_elems : 1)) != 0) {
+        || dev_malloc(&D->U, (size_t)elem * (D->U_elems > 0 ? D->U_elems : 1)) != 0
+        || dev_malloc(&D->Inv, (size_t)elem * (D->I_elems > 0 ? D->I_elems : 1)) != 0) {
       tamd_dev_free_numeric(F);
       return -1;
     }
@@ -491,35 +597,66 @@ template <typename T>
 static int run_solve(tamd_factor* F, int nrhs, double* dX, int ldx, const double* dB, int ldb)
 {
   tamd_dev* D = F->dev;
-  const tamd_plan* P = &F->plan;
   hipStream_t st = D->stream;
+  const int n = F->n;
   if (!D->W || D->W_rhs < nrhs) {
     free_ptr(D->W);
-    D->W = nullptr;
+    free_ptr(D->Y);
+    D->W = D->Y = nullptr;
     int cap = std::max(nrhs, 16);
-    if (dev_malloc(&D->W, sizeof(T) * (size_t)D->W_per_rhs * cap) != 0) return -1;
+    free_ptr(D->Part);
+    D->Part = nullptr;
+    if (dev_malloc(&D->W, sizeof(T) * (size_t)D->W_per_rhs * cap) != 0
+        || dev_malloc(&D->Y, sizeof(T) * (size_t)n * cap) != 0
+        || dev_malloc(&D->Part, sizeof(T) * 64 * 16 * (size_t)std::max(D->part_slots, 1LL)) != 0)
+      return -1;
     D->W_rhs = cap;
   }
   T* W = (T*)D->W;
+  T* Y = (T*)D->Y;
+  T* Part = (T*)D->Part;
   const T* L = (const T*)D->L;
-  hipLaunchKernelGGL(k_solve_init<T>, dim3(D->n_fronts), dim3(128), 0, st, D->d_fronts, D->d_rows,
-                     dB, ldb, W, nrhs);
-  for (int l = 0; l < P->n_levels; l++) {
-    int b0 = D->lvl_ptr[l], b1 = D->lvl_ptr[l + 1];
-    hipLaunchKernelGGL(k_solve_fwd<T>, dim3(b1 - b0), dim3(256), 0, st, D->d_fronts,
-                       D->d_solve_list + b0, L, W, nrhs);
-    for (size_t q = 0; q + 1 < D->sext_ptr.size(); q++) {
-      if (D->sext_lvl[q] != l) continue;
-      int c0 = D->sext_ptr[q], c1 = D->sext_ptr[q + 1];
-      if (c1 > c0)
-        hipLaunchKernelGGL(k_solve_ext<T>, dim3(c1 - c0), dim3(128), 0, st, D->d_fronts,
-                           D->d_solve_ext + c0, D->d_rel, W, nrhs);
+  const T* Iv = (const T*)D->Inv;
+  for (const Op& op : D->sv_ops) {
+    const PanelTask* tk = D->d_sv_tasks + op.start;
+    switch (op.kind) {
+      case SV_INIT:
+        hipLaunchKernelGGL(k_solve_init<T>, dim3(op.count), dim3(128), 0, st, D->d_fronts, D->d_rows,
+                           dB, ldb, W, nrhs);
+        break;
+      case SV_FSTEP:
+        hipLaunchKernelGGL(k_fwd_step<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk,
+                           D->d_rows, L, Iv, W, Y, n, nrhs);
+        break;
+      case SV_FRECT:
+        hipLaunchKernelGGL(k_fwd_rect<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk,
+                           D->d_rows, L, (const T*)Y, n, Part, nrhs);
+        break;
+      case SV_FRED:
+        hipLaunchKernelGGL(k_fwd_rect_reduce<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk,
+                           (const T*)Part, W, nrhs);
+        break;
+      case SV_EXT:
+        hipLaunchKernelGGL(k_solve_ext<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk, D->d_rel,
+                           W, nrhs);
+        break;
+      case SV_GATHER:
+        hipLaunchKernelGGL(k_bwd_gather<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk,
+                           D->d_rows, (const double*)dX, ldx, W, nrhs);
+        break;
+      case SV_BCOLS:
+        hipLaunchKernelGGL(k_bwd_cols<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk, L,
+                           (const T*)W, Part, nrhs);
+        break;
+      case SV_BRED:
+        hipLaunchKernelGGL(k_bwd_cols_reduce<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk,
+                           D->d_rows, (const T*)Part, Y, n, nrhs);
+        break;
+      case SV_BSTEP:
+        hipLaunchKernelGGL(k_bwd_step<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk, D->d_rows,
+                           L, Iv, Y, n, dX, ldx, nrhs);
+        break;
     }
-  }
-  for (int l = P->n_levels - 1; l >= 0; l--) {
-    int b0 = D->lvl_ptr[l], b1 = D->lvl_ptr[l + 1];
-    hipLaunchKernelGGL(k_solve_bwd<T>, dim3(b1 - b0), dim3(256), 0, st, D->d_fronts,
-                       D->d_solve_list + b0, D->d_rows, L, W, dX, ldx, nrhs);
   }
   HIPCHECK(hipGetLastError());
   return 0;

@@ -29,111 +29,167 @@ __global__ void k_assemble(int ncol, const int* __restrict__ colptr,
   for (int ip = colptr[j]; ip < e; ip++) L[dest[ip]] += (T)vals[ip];
 }
 
-// Right-looking unblocked Cholesky of the first `ncols` columns of an
-// nrows x ncols block held in LDS (ld 65), one row per lane.  Rows beyond
-// ncols are the sub-diagonal part and receive the triangular solve.
-// Returns nonzero in *bad when a pivot is not positive.
+// Right-looking unblocked Cholesky of an nrows x ncols panel (both <= 64),
+// fused with the inversion of the factored diagonal block (X = L11^{-1}).
+// The panel is staged through LDS into registers: thread (wave w, lane j)
+// owns rows i = w + 4q (q < 16) of column j of both A and X, so the rank-1
+// updates of every step run in registers and LDS only carries broadcasts:
+// the pivot column (colk), the pivot row of X (xrow) and the reciprocal of
+// the pivot.  Rows beyond ncols receive the triangular solve.  On return the
+// factor and the inverse are back in A and X (LDS, ld 65).
 template <typename T>
-__device__ void lds_panel_chol(T* P, int nrows, int ncols, int lane, int* bad)
+__device__ __forceinline__ void lds_chol_inv(T* __restrict__ A, T* __restrict__ X,
+                                             T* __restrict__ colk, T* __restrict__ xrow,
+                                             int nrows, int ncols, int* bad)
 {
+  const int tid = threadIdx.x, j = tid & 63, w = tid >> 6;
+  T a[16], x[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int i = w + 4 * q;
+    a[q] = (i < nrows && j < ncols) ? A[i + j * 65] : (T)0;
+    x[q] = (i == j) ? (T)1 : (T)0;
+  }
+  __syncthreads();
   for (int k = 0; k < ncols; k++) {
-    if (lane == k) {
-      T d = P[k + k * 65];
-      if (!(d > (T)0)) { *bad = 1; d = (T)1; }
-      P[k + k * 65] = sqrt(d);
+    const int qk = k >> 2, wk = k & 3;
+    // pivot: row k lives in wave wk, register qk, lane k
+    if (w == wk && j == k) {
+      T p = a[0];
+#pragma unroll
+      for (int q = 1; q < 16; q++)
+        if (q == qk) p = a[q];
+      if (!(p > (T)0)) { *bad = 1; p = (T)1; }
+      const T d = sqrt(p);
+      colk[64] = d;
+      colk[65] = (T)1 / d;
     }
     __syncthreads();
-    T piv = P[k + k * 65];
-    if (lane > k && lane < nrows) P[lane + k * 65] /= piv;
+    const T dk = colk[64], rdk = colk[65];
+    if (j == k) {  // column k: scale rows below, publish
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int i = w + 4 * q;
+        if (i == k) a[q] = dk;
+        if (i > k && i < nrows) {
+          a[q] = a[q] * rdk;
+          colk[i] = a[q];
+        }
+      }
+    }
+    if (w == wk && j <= k) {  // row k of X: scale, publish
+#pragma unroll
+      for (int q = 0; q < 16; q++)
+        if (q == qk) {
+          x[q] = x[q] * rdk;
+          xrow[j] = x[q];
+        }
+    }
     __syncthreads();
-    if (lane > k && lane < nrows) {
-      T lk = P[lane + k * 65];
-      int jmax = lane < ncols ? lane : ncols - 1;
-      for (int j = k + 1; j <= jmax; j++) P[lane + j * 65] -= lk * P[j + k * 65];
+    {
+      const T lj = (j > k && j < 64) ? colk[j] : (T)0;
+      const T xk = j <= k ? xrow[j] : (T)0;
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int i = w + 4 * q;
+        if (i > k && i < nrows) {
+          const T ci = colk[i];
+          if (j > k && j < ncols && j <= i) a[q] -= ci * lj;
+          if (i < ncols && j <= k) x[q] -= ci * xk;
+        }
+      }
     }
     __syncthreads();
   }
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int i = w + 4 * q;
+    A[i + j * 65] = a[q];
+    X[i + j * 65] = x[q];
+  }
+  __syncthreads();
 }
 
-// Whole front (m <= 64) in one wave: POTRF + TRSM in LDS, SYRK into U.
+// Store X (ncols x ncols, lower) row-major zero padded to 64 x 64.
 template <typename T>
-__global__ __launch_bounds__(64) void k_front_small(const FrontDesc* __restrict__ fr,
-                                                    const int* __restrict__ list,
-                                                    T* __restrict__ L, T* __restrict__ U,
-                                                    int* __restrict__ info)
+__device__ __forceinline__ void store_inv(const T* X, int kb, T* __restrict__ Inv)
 {
-  __shared__ T P[64 * 65];
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    int c = e & 63, i = e >> 6;
+    Inv[i * 64 + c] = (i < kb && c < kb && c <= i) ? X[i + c * 65] : (T)0;
+  }
+}
+
+// Whole front (m <= 64): POTRF + TRSM + inverse in LDS, SYRK into U.
+template <typename T>
+__global__ __launch_bounds__(256) void k_front_small(const FrontDesc* __restrict__ fr,
+                                                     const int* __restrict__ list,
+                                                     T* __restrict__ L, T* __restrict__ U,
+                                                     T* __restrict__ Inv, int* __restrict__ info)
+{
+  __shared__ T A[64 * 65];
+  __shared__ T X[64 * 65];
+  __shared__ T colk[66];
+  __shared__ T xrow[64];
   __shared__ int bad;
   const FrontDesc d = fr[list[blockIdx.x]];
-  const int s = d.s, m = d.m, u = m - s, lane = threadIdx.x;
+  const int s = d.s, m = d.m, u = m - s, tid = threadIdx.x;
   T* gp = L + d.offL;
-  if (lane == 0) bad = 0;
-  for (int c = 0; c < s; c++)
-    if (lane < m) P[lane + c * 65] = gp[lane + (long long)c * m];
+  if (tid == 0) bad = 0;
+  for (int e = tid; e < 64 * s; e += 256) {
+    int i = e & 63, c = e >> 6;
+    if (i < m) A[i + c * 65] = gp[i + (long long)c * m];
+  }
   __syncthreads();
-  lds_panel_chol(P, m, s, lane, &bad);
-  for (int c = 0; c < s; c++)
-    if (lane < m && lane >= c) gp[lane + (long long)c * m] = P[lane + c * 65];
-  if (u > 0 && lane < u) {
+  lds_chol_inv(A, X, colk, xrow, m, s, &bad);
+  for (int e = tid; e < 64 * s; e += 256) {
+    int i = e & 63, c = e >> 6;
+    if (i < m && i >= c) gp[i + (long long)c * m] = A[i + c * 65];
+  }
+  store_inv(X, s, Inv + d.offI);
+  if (u > 0) {
     T* gu = U + d.offU;
-    const int i = lane;
-    for (int j = 0; j <= i; j++) {
+    for (int e = tid; e < u * u; e += 256) {
+      int i = e % u, j = e / u;
+      if (i < j) continue;
       T acc = (T)0;
-      for (int k = 0; k < s; k++) acc += P[s + i + k * 65] * P[s + j + k * 65];
+      for (int k = 0; k < s; k++) acc += A[s + i + k * 65] * A[s + j + k * 65];
       gu[i + (long long)j * u] -= acc;
     }
   }
-  if (lane == 0 && bad) atomicOr(info, 1);
+  if (tid == 0 && bad) atomicOr(info, 1);
 }
 
-// Cholesky of one kb x kb diagonal block of a large front's panel.
+// Cholesky of one kb x kb diagonal block of a large front's panel, and the
+// inverse of the factored block.
 template <typename T>
-__global__ __launch_bounds__(64) void k_potrf_diag(const FrontDesc* __restrict__ fr,
-                                                   const PanelTask* __restrict__ tasks,
-                                                   T* __restrict__ L, int* __restrict__ info)
+__global__ __launch_bounds__(256) void k_potrf_diag(const FrontDesc* __restrict__ fr,
+                                                    const PanelTask* __restrict__ tasks,
+                                                    T* __restrict__ L, T* __restrict__ Inv,
+                                                    int* __restrict__ info)
 {
-  __shared__ T D[64 * 65];
+  __shared__ T A[64 * 65];
+  __shared__ T X[64 * 65];
+  __shared__ T colk[66];
+  __shared__ T xrow[64];
   __shared__ int bad;
   const PanelTask t = tasks[blockIdx.x];
   const FrontDesc d = fr[t.front];
-  const int m = d.m, lane = threadIdx.x, kb = t.kb;
+  const int m = d.m, tid = threadIdx.x, kb = t.kb;
   T* gp = L + d.offL + t.k0 + (long long)t.k0 * m;
-  if (lane == 0) bad = 0;
-  for (int c = 0; c < kb; c++)
-    if (lane < kb) D[lane + c * 65] = gp[lane + (long long)c * m];
-  __syncthreads();
-  lds_panel_chol(D, kb, kb, lane, &bad);
-  for (int c = 0; c < kb; c++)
-    if (lane < kb && lane >= c) gp[lane + (long long)c * m] = D[lane + c * 65];
-  if (lane == 0 && bad) atomicOr(info, 1);
-}
-
-// Rows r0..r0+nr-1 of columns k0..k0+kb-1:  X <- X * L_kk^{-T}.
-template <typename T>
-__global__ __launch_bounds__(64) void k_trsm_rows(const FrontDesc* __restrict__ fr,
-                                                  const PanelTask* __restrict__ tasks,
-                                                  T* __restrict__ L)
-{
-  __shared__ T Lk[64 * 65];
-  __shared__ T X[64 * 65];
-  const PanelTask t = tasks[blockIdx.x];
-  const FrontDesc d = fr[t.front];
-  const int m = d.m, lane = threadIdx.x, kb = t.kb, nr = t.nr;
-  const T* gl = L + d.offL + t.k0 + (long long)t.k0 * m;
-  T* gx = L + d.offL + t.r0 + (long long)t.k0 * m;
-  for (int c = 0; c < kb; c++) {
-    if (lane < kb) Lk[lane + c * 65] = gl[lane + (long long)c * m];
-    if (lane < nr) X[lane + c * 65] = gx[lane + (long long)c * m];
+  if (tid == 0) bad = 0;
+  for (int e = tid; e < 64 * kb; e += 256) {
+    int i = e & 63, c = e >> 6;
+    if (i < kb) A[i + c * 65] = gp[i + (long long)c * m];
   }
   __syncthreads();
-  if (lane < nr) {
-    for (int j = 0; j < kb; j++) {
-      T v = X[lane + j * 65];
-      for (int q = 0; q < j; q++) v -= X[lane + q * 65] * Lk[j + q * 65];
-      X[lane + j * 65] = v / Lk[j + j * 65];
-    }
-    for (int c = 0; c < kb; c++) gx[lane + (long long)c * m] = X[lane + c * 65];
+  lds_chol_inv(A, X, colk, xrow, kb, kb, &bad);
+  for (int e = tid; e < 64 * kb; e += 256) {
+    int i = e & 63, c = e >> 6;
+    if (i < kb && i >= c) gp[i + (long long)c * m] = A[i + c * 65];
   }
+  store_inv(X, kb, Inv + d.offI + (long long)(t.k0 / 64) * 4096);
+  if (tid == 0 && bad) atomicOr(info, 1);
 }
 
 // ---- MFMA tile GEMM ---------------------------------------------------------
@@ -164,12 +220,15 @@ template <> struct Mfma<float> {
   static __device__ int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-#define TK 16
+#define TK 32
 #define TPAD 4
 
+// K is consumed in chunks of 32: the next chunk is fetched from global memory
+// into registers while the MFMAs run on the current chunk in LDS.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ tasks,
-                                                   const T* Lsrc, T* L, T* U)
+                                                   const T* Lsrc, T* L, T* U,
+                                                   const T* __restrict__ Inv)
 {
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T As[TK][64 + TPAD];
@@ -178,21 +237,33 @@ __global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const T* A = Lsrc + t.a_off;
-  const T* B = Lsrc + t.b_off;
+  const bool binv = (t.flags & 4) != 0;
+  const T* B = binv ? Inv + t.b_off : Lsrc + t.b_off;
   acc_t acc[2][2];
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++)
       for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
 
-  const int li = tid & 63, lk = tid >> 6;  // loader: row li, k lk + 4q
+  const int li = tid & 63, lk = tid >> 6;  // loader: row li, k = lk + 4q
+  T ra[TK / 4], rb[TK / 4];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int q = 0; q < TK / 4; q++) {
+      const int k = kc + lk + 4 * q;
+      const bool kin = k < t.K;
+      ra[q] = (kin && li < t.M) ? A[li + (long long)k * t.ld] : (T)0;
+      rb[q] = (kin && li < t.N) ? (binv ? B[li * 64 + k] : B[li + (long long)k * t.ld]) : (T)0;
+    }
+  };
+  fetch(0);
   for (int kc = 0; kc < t.K; kc += TK) {
-    for (int q = 0; q < 4; q++) {
-      int k = lk + 4 * q;
-      bool kin = kc + k < t.K;
-      As[k][li] = (kin && li < t.M) ? A[li + (long long)(kc + k) * t.ld] : (T)0;
-      Bs[k][li] = (kin && li < t.N) ? B[li + (long long)(kc + k) * t.ld] : (T)0;
+#pragma unroll
+    for (int q = 0; q < TK / 4; q++) {
+      As[lk + 4 * q][li] = ra[q];
+      Bs[lk + 4 * q][li] = rb[q];
     }
     __syncthreads();
+    if (kc + TK < t.K) fetch(kc + TK);
 #pragma unroll
     for (int kk = 0; kk < TK; kk += 4) {
       const int kr = kk + (lane >> 4), c16 = lane & 15;
@@ -213,8 +284,10 @@ __global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ 
       for (int r = 0; r < 4; r++) {
         int i = wr + a * 16 + (lane & 15);
         int j = wc + b * 16 + Mfma<T>::drow(lane, r);
-        if (i < t.M && j < t.N && (!diag || i >= j))
-          C[i + (long long)j * t.ldc] -= acc[a][b][r];
+        if (i < t.M && j < t.N && (!diag || i >= j)) {
+          if (binv) C[i + (long long)j * t.ldc] = acc[a][b][r];
+          else C[i + (long long)j * t.ldc] -= acc[a][b][r];
+        }
       }
 }
 
@@ -303,10 +376,9 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
   template __global__ void k_fill<T>(T*, long long);                                   \
   template __global__ void k_assemble<T>(int, const int*, const long long*,             \
                                          const double*, T*);                            \
-  template __global__ void k_front_small<T>(const FrontDesc*, const int*, T*, T*, int*); \
-  template __global__ void k_potrf_diag<T>(const FrontDesc*, const PanelTask*, T*, int*); \
-  template __global__ void k_trsm_rows<T>(const FrontDesc*, const PanelTask*, T*);      \
-  template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*);           \
+  template __global__ void k_front_small<T>(const FrontDesc*, const int*, T*, T*, T*, int*); \
+  template __global__ void k_potrf_diag<T>(const FrontDesc*, const PanelTask*, T*, T*, int*); \
+  template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*);
 TAMD_INST(double)
 TAMD_INST(float)

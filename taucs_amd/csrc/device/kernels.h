@@ -20,6 +20,7 @@ struct FrontDesc {
   long long offL;   // panel offset (elements)
   long long offU;   // update block offset
   long long offW;   // solve workspace offset (elements per rhs column: m)
+  long long offI;   // inverses of the 64x64 diagonal blocks (row-major, 4096 each)
   int s, m;         // columns, rows (m = s + u)
   int parent;       // index into the front table, -1 at a root
   int rel;          // offset of this front's update rows in the rel array
@@ -29,17 +30,19 @@ struct FrontDesc {
 
 struct GemmTask {   // C[i,j] -= sum_k A[i,k] * B[j,k] on one 64x64 tile
   long long a_off;  // A(i,k) = L[a_off + i + k*ld]
-  long long b_off;  // B(j,k) = L[b_off + j + k*ld]
+  long long b_off;  // B(j,k) = L[b_off + j + k*ld]  (flag 4: Inv[b_off + j*64 + k])
   long long c_off;  // C(i,j) = C[c_off + i + j*ldc]
   int ld, ldc;
   int M, N, K;
-  int flags;        // 1: C lives in U (else in L); 2: diagonal tile, lower part only
+  int flags;        // 1: C lives in U (else in L); 2: diagonal tile, lower part only;
+                    // 4: B is a row-major inverse block and C = A B^T is stored
+                    //    (the panel triangular solve X <- X L_kk^{-T}, in place)
 };
 
 struct PanelTask {  // one diagonal block, or one row block of the panel
   int front;
   int k0, kb;       // columns k0 .. k0+kb-1 of the panel
-  int r0, nr;       // rows r0 .. r0+nr-1 (trsm only)
+  int r0, nr;       // rows (or columns) r0 .. r0+nr-1
 };
 
 struct EATask {     // extend-add columns j0..j1-1 of a child's update block
