@@ -86,6 +86,28 @@ int taucs_amd_solve_refined(void* L, taucs_ccs_matrix* A, int nrhs, double* X, i
                             const double* B, int ldb, int method, double tol, int maxit,
                             taucs_amd_refine_info* info);
 
+/* ---- many independent systems in the same launches ---------------------- */
+/* A[q] are the permuted matrices P A P^T of `count` independent systems.  Each
+   system keeps the reference's own symbolic analysis; the numeric phase runs
+   the systems' fronts level by level in shared launches (hipGraph-replayed).
+   Vectors are concatenated: system q owns rows xoff[q] .. xoff[q+1]-1. */
+void* taucs_amd_batch_factor_llt_mf(int count, taucs_ccs_matrix** A, int precision);
+int   taucs_amd_batch_numeric(void* B, taucs_ccs_matrix** A, int precision);
+int   taucs_amd_batch_solve(void* B, int nrhs, double* X, int ldx, const double* Bv, int ldb);
+int   taucs_amd_batch_solve_device(void* B, int nrhs, double* dX, int ldx, const double* dB,
+                                   int ldb);
+int   taucs_amd_batch_solve_refined(void* B, int nrhs, double* X, int ldx, const double* Bv,
+                                    int ldb, int method, double tol, int maxit,
+                                    taucs_amd_refine_info* info);
+int   taucs_amd_batch_info(void* B, int* count, int* n_total, long long* xoff,
+                           double* fund_flops, double* relax_flops, int* n_levels,
+                           double* t_numeric);
+void* taucs_amd_batch_system_factor(void* B, int q);
+void  taucs_amd_batch_free(void* B);
+
+/* hipGraph capture of the numeric schedule and of the solves (default on) */
+void  taucs_amd_set_graphs(int on);
+
 /* ---- device buffers (for right-hand sides kept in HBM) ------------------- */
 void* taucs_amd_device_malloc(size_t bytes);
 void  taucs_amd_device_free(void* p);
@@ -106,6 +128,7 @@ typedef struct {
 } taucs_amd_numeric_timing;
 void taucs_amd_set_profiling(int on);
 int  taucs_amd_get_numeric_timing(void* L, taucs_amd_numeric_timing* t);
+int  taucs_amd_batch_get_timing(void* B, taucs_amd_numeric_timing* t);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,19 @@ _sigs = {
                                           C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int,
                                           P(RefineInfo)]),
     "taucs_amd_set_profiling": (None, [C.c_int]),
+    "taucs_amd_set_graphs": (None, [C.c_int]),
+    "taucs_amd_batch_factor_llt_mf": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int]),
+    "taucs_amd_batch_numeric": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "taucs_amd_batch_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    "taucs_amd_batch_solve_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                               C.c_int]),
+    "taucs_amd_batch_solve_refined": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                                C.c_int, C.c_int, C.c_double, C.c_int, P(RefineInfo)]),
+    "taucs_amd_batch_info": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int), C.c_void_p, P(C.c_double),
+                                       P(C.c_double), P(C.c_int), P(C.c_double)]),
+    "taucs_amd_batch_system_factor": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "taucs_amd_batch_free": (None, [C.c_void_p]),
+    "taucs_amd_batch_get_timing": (C.c_int, [C.c_void_p, P(NumericTiming)]),
     "taucs_amd_device_malloc": (C.c_void_p, [C.c_size_t]),
     "taucs_amd_device_free": (None, [C.c_void_p]),
     "taucs_amd_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -383,6 +396,85 @@ class DeviceArray:
         if self.ptr:
             lib.taucs_amd_device_free(self.ptr)
             self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent systems (permuted matrices) factored in shared launches."""
+
+    def __init__(self, mats, precision=FP64):
+        self.mats = list(mats)
+        arr = (C.c_void_p * len(self.mats))(*[C.cast(m.ptr, C.c_void_p) for m in self.mats])
+        self._arr = arr
+        h = lib.taucs_amd_batch_factor_llt_mf(len(self.mats), arr, precision)
+        if not h:
+            raise RuntimeError("batched factorization failed")
+        self.h = C.c_void_p(h)
+        cnt, nt, lv = C.c_int(), C.c_int(), C.c_int()
+        ff, rf, tn = C.c_double(), C.c_double(), C.c_double()
+        self.xoff = np.zeros(len(self.mats) + 1, dtype=np.int64)
+        lib.taucs_amd_batch_info(self.h, C.byref(cnt), C.byref(nt), _dptr(self.xoff), C.byref(ff),
+                                 C.byref(rf), C.byref(lv), C.byref(tn))
+        self.n_total = nt.value
+        self.fund_flops = ff.value
+        self.relax_flops = rf.value
+        self.n_levels = lv.value
+
+    def t_numeric(self):
+        tn = C.c_double()
+        lib.taucs_amd_batch_info(self.h, None, None, None, None, None, None, C.byref(tn))
+        return tn.value
+
+    def numeric(self, mats=None, precision=FP64):
+        mats = list(mats) if mats is not None else self.mats
+        arr = (C.c_void_p * len(mats))(*[C.cast(m.ptr, C.c_void_p) for m in mats])
+        return lib.taucs_amd_batch_numeric(self.h, arr, precision)
+
+    def solve(self, b):
+        b = np.asarray(b, dtype=np.float64)
+        one = b.ndim == 1
+        B = np.asfortranarray(b.reshape(b.shape[0], -1))
+        X = np.empty_like(B, order="F")
+        if lib.taucs_amd_batch_solve(self.h, B.shape[1], _dptr(X), B.shape[0], _dptr(B), B.shape[0]):
+            raise RuntimeError("batched solve failed")
+        return X[:, 0].copy() if one else X
+
+    def solve_device(self, dX, dB, nrhs):
+        if lib.taucs_amd_batch_solve_device(self.h, nrhs, dX, self.n_total, dB, self.n_total):
+            raise RuntimeError("batched device solve failed")
+
+    def solve_refined(self, b, method=0, tol=1e-14, maxit=100):
+        b = np.asarray(b, dtype=np.float64)
+        one = b.ndim == 1
+        B = np.asfortranarray(b.reshape(b.shape[0], -1))
+        X = np.empty_like(B, order="F")
+        info = RefineInfo()
+        if lib.taucs_amd_batch_solve_refined(self.h, B.shape[1], _dptr(X), B.shape[0], _dptr(B),
+                                             B.shape[0], method, tol, maxit, C.byref(info)):
+            raise RuntimeError("batched refined solve failed")
+        return (X[:, 0].copy() if one else X), info
+
+    def system_supernodes(self, q):
+        f = Factor(lib.taucs_amd_batch_system_factor(self.h, q))
+        try:
+            return f.supernodes()
+        finally:
+            f.h = None  # owned by the batch
+
+    def timing(self):
+        t = NumericTiming()
+        lib.taucs_amd_batch_get_timing(self.h, C.byref(t))
+        return t
+
+    def free(self):
+        if self.h:
+            lib.taucs_amd_batch_free(self.h)
+            self.h = None
 
     def __del__(self):
         try:

@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(ROOT, "_build", "obj")
 LIB = os.path.join(HERE, "libtaucs_amd.so")
 
-HOST_SOURCES = ["util.c", "ccs.c", "mmd.c", "symbolic.c", "order.c", "factor.c", "linsolve.c"]
+HOST_SOURCES = ["util.c", "ccs.c", "mmd.c", "symbolic.c", "order.c", "factor.c", "linsolve.c",
+                "batch.c"]
 DEVICE_SOURCES = ["factor_kernels.hip", "solve_kernels.hip", "driver.hip", "refine.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -74,7 +74,7 @@ struct tamd_dev {
   int* d_info = nullptr;
   std::vector<Op> ops;
   std::vector<int> lvl_ptr;      // fronts of level l: [lvl_ptr[l], lvl_ptr[l+1])
-  int n_fronts = 0, ncol = 0, nnz = 0;
+  int n_fronts = 0, ncol = 0, nnz = 0, n_levels = 0;
   long long W_per_rhs = 0;
   double gemm_flops = 0, assembly_bytes = 0;
   int gemm_launches = 0;
@@ -96,7 +96,38 @@ struct tamd_dev {
   double* d_b = nullptr;
   int xb_rhs = 0;
   taucs_amd_numeric_timing timing;
+  // captured launch sequences (replayed while the buffers they name live)
+  hipGraphExec_t num_graph = nullptr;
+  int num_graph_key = -1;
+  struct SolveGraph {
+    hipGraphExec_t exec = nullptr;
+    int nrhs = 0, prec = -1, ldx = 0, ldb = 0;
+    const void* x = nullptr;
+    const void* b = nullptr;
+    unsigned long long used = 0;
+  };
+  SolveGraph sv[4];
+  unsigned long long sv_clock = 0;
 };
+
+static int g_graphs = 1;
+extern "C" void taucs_amd_set_graphs(int on) { g_graphs = on; }
+
+static void drop_solve_graphs(tamd_dev* D)
+{
+  for (auto& g : D->sv) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = tamd_dev::SolveGraph();
+  }
+}
+
+static void drop_graphs(tamd_dev* D)
+{
+  if (D->num_graph) (void)hipGraphExecDestroy(D->num_graph);
+  D->num_graph = nullptr;
+  D->num_graph_key = -1;
+  drop_solve_graphs(D);
+}
 
 static int dev_malloc(void** p, size_t bytes)
 {
@@ -139,6 +170,7 @@ extern "C" void tamd_dev_free_numeric(tamd_factor* F)
   tamd_dev* D = F ? F->dev : nullptr;
   if (!D) return;
   if (D->stream) (void)hipStreamSynchronize(D->stream);
+  drop_graphs(D);
   free_ptr(D->L);
   free_ptr(D->U);
   free_ptr(D->Inv);
@@ -179,48 +211,95 @@ static double tile_flops(const GemmTask& g)
 static const int SMALL_M = 64;   // fronts with m <= 64 run whole in one wave
 static const int NB = 64;        // panel block width of large fronts
 
-extern "C" int tamd_dev_setup(tamd_factor* F)
-{
-  const tamd_symbolic* S = &F->sym;
-  const tamd_plan* P = &F->plan;
-  if (!tamd_dev_available()) {
-    taucs_printf((char*)"taucs_amd: no GPU device available; the numeric phase runs only on the GPU\n");
-    return -1;
-  }
-  tamd_dev* D = new tamd_dev();
-  F->dev = D;
-  if (hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking) != hipSuccess) {
-    tamd_dev_free(F);
-    return -1;
-  }
-  const int nf = P->n_live;
-  D->n_fronts = nf;
-  std::vector<int> idx_of(P->n_sn + 1, -1);
-  for (int k = 0; k < nf; k++) idx_of[P->order[k]] = k;
+// One independent system of a (possibly batched) device schedule.
+struct SysView {
+  const tamd_symbolic* S;
+  const tamd_plan* P;
+  long long xoff;             // first row of this system in the concatenated vectors
+};
 
+// Builds the device schedule of one or more independent systems.  Fronts of
+// level l of every system share the launches of level l; each system keeps
+// its own arenas (offsets), row numbering (shifted by xoff) and tree.
+static int dev_build(tamd_dev* D, const std::vector<SysView>& sys)
+{
+  int nlev = 0, nf = 0;
+  long long nnz_tot = 0;
+  for (const SysView& v : sys) {
+    nlev = std::max(nlev, v.P->n_levels);
+    nf += v.P->n_live;
+    nnz_tot += v.P->a_nnz;
+  }
+  D->n_fronts = nf;
   std::vector<FrontDesc> fr(nf);
-  std::vector<int> rows;
-  long long offW = 0, offI = 0;
-  for (int k = 0; k < nf; k++) {
-    int sn = P->order[k];
-    FrontDesc d;
-    memset(&d, 0, sizeof(d));
-    d.offL = P->off_L[sn];
-    d.offU = P->off_U[sn];
-    d.offW = offW;
-    d.s = S->sn_size[sn];
-    d.m = S->sn_up_size[sn];
-    d.parent = P->sn_parent[sn] < P->n_sn ? idx_of[P->sn_parent[sn]] : -1;
-    d.rel = P->rel_ptr[sn];
-    d.rows = (int)rows.size();
-    d.offI = offI;
-    rows.insert(rows.end(), S->sn_struct[sn], S->sn_struct[sn] + d.m);
-    offW += d.m;
-    offI += (long long)((d.s + 63) / 64) * 4096;
-    fr[k] = d;
+  std::vector<int> sibr(nf), rows, rel;
+  std::vector<int> lvl(nlev + 1, 0);
+  std::vector<long long> adest;
+  adest.reserve(nnz_tot);
+  // global index of (system, plan position), grouped by level across systems
+  std::vector<std::vector<int>> gidx(sys.size());
+  {
+    int g = 0;
+    for (int l = 0; l < nlev; l++) {
+      lvl[l] = g;
+      for (size_t q = 0; q < sys.size(); q++) {
+        const tamd_plan* P = sys[q].P;
+        if (gidx[q].empty()) gidx[q].assign(P->n_live, -1);
+        if (l >= P->n_levels) continue;
+        for (int k = P->level_ptr[l]; k < P->level_ptr[l + 1]; k++) gidx[q][k] = g++;
+      }
+    }
+    lvl[nlev] = g;
+  }
+  long long Lbase = 0, Ubase = 0, offW = 0, offI = 0;
+  for (size_t q = 0; q < sys.size(); q++) {
+    const tamd_symbolic* S = sys[q].S;
+    const tamd_plan* P = sys[q].P;
+    std::vector<int> idx_of(P->n_sn + 1, -1);
+    for (int k = 0; k < P->n_live; k++) idx_of[P->order[k]] = gidx[q][k];
+    const int relbase = (int)rel.size();
+    for (int k = 0; k < P->n_live; k++) {
+      int sn = P->order[k];
+      FrontDesc d;
+      memset(&d, 0, sizeof(d));
+      d.offL = Lbase + P->off_L[sn];
+      d.offU = Ubase + P->off_U[sn];
+      d.s = S->sn_size[sn];
+      d.m = S->sn_up_size[sn];
+      d.parent = P->sn_parent[sn] < P->n_sn ? idx_of[P->sn_parent[sn]] : -1;
+      d.rel = relbase + P->rel_ptr[sn];
+      fr[gidx[q][k]] = d;
+      sibr[gidx[q][k]] = P->sib_rank[sn];
+    }
+    int nrel = P->rel_ptr[P->n_sn] > 0 ? P->rel_ptr[P->n_sn] : 0;
+    rel.insert(rel.end(), P->rel, P->rel + nrel);
+    for (int e = 0; e < P->a_nnz; e++) adest.push_back(Lbase + P->a_dest[e]);
+    Lbase += P->L_elems;
+    Ubase += P->U_elems;
+  }
+  // row lists, workspace and inverse offsets in global (level-grouped) order
+  for (int g = 0; g < nf; g++) fr[g].rows = -1;
+  for (size_t q = 0; q < sys.size(); q++) {
+    const tamd_symbolic* S = sys[q].S;
+    const tamd_plan* P = sys[q].P;
+    for (int k = 0; k < P->n_live; k++) {
+      int sn = P->order[k], g = gidx[q][k];
+      fr[g].rows = (int)rows.size();
+      for (int e = 0; e < S->sn_up_size[sn]; e++) rows.push_back((int)(sys[q].xoff + S->sn_struct[sn][e]));
+    }
+  }
+  for (int g = 0; g < nf; g++) {
+    fr[g].offW = offW;
+    fr[g].offI = offI;
+    offW += fr[g].m;
+    offI += (long long)((fr[g].s + 63) / 64) * 4096;
   }
   D->W_per_rhs = offW;
   D->I_elems = offI;
+  D->L_elems = Lbase;
+  D->U_elems = Ubase;
+  D->nnz = (int)nnz_tot;
+  D->n_levels = nlev;
 
   // factorization task lists, level by level
   std::vector<int> small;
@@ -228,13 +307,13 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
   std::vector<GemmTask> trsm;
   std::vector<GemmTask> gemm;
   std::vector<EATask> ea;
-  D->lvl_ptr.assign(P->level_ptr, P->level_ptr + P->n_levels + 1);
+  D->lvl_ptr = lvl;
   D->ops.clear();
   D->ops.push_back({TAMD_OP_ASSEMBLE, 0, 0, -1});
   D->gemm_flops = 0;
-  D->assembly_bytes = 12.0 * P->a_nnz;
-  for (int l = 0; l < P->n_levels; l++) {
-    int b0 = P->level_ptr[l], b1 = P->level_ptr[l + 1];
+  D->assembly_bytes = 12.0 * (double)nnz_tot;
+  for (int l = 0; l < nlev; l++) {
+    int b0 = lvl[l], b1 = lvl[l + 1];
     int st = (int)small.size();
     int maxsteps = 0;
     for (int k = b0; k < b1; k++) {
@@ -313,13 +392,13 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
     {
       int maxrank = -1;
       for (int k = b0; k < b1; k++)
-        if (fr[k].parent >= 0) maxrank = std::max(maxrank, P->sib_rank[P->order[k]]);
+        if (fr[k].parent >= 0) maxrank = std::max(maxrank, sibr[k]);
       for (int rk = 0; rk <= maxrank; rk++) {
         int e0 = (int)ea.size();
         for (int k = b0; k < b1; k++) {
           const FrontDesc& d = fr[k];
           int u = d.m - d.s;
-          if (d.parent < 0 || u == 0 || P->sib_rank[P->order[k]] != rk) continue;
+          if (d.parent < 0 || u == 0 || sibr[k] != rk) continue;
           D->assembly_bytes += 24.0 * ((double)u * (u + 1) / 2);
           // chunks of about 16K entries
           int j = 0;
@@ -346,8 +425,8 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
   };
   const int SB = 64, CH = 512;
   D->part_slots = 0;
-  for (int l = 0; l < P->n_levels; l++) {
-    int b0 = P->level_ptr[l], b1 = P->level_ptr[l + 1];
+  for (int l = 0; l < nlev; l++) {
+    int b0 = lvl[l], b1 = lvl[l + 1];
     int maxb = 0;
     for (int k = b0; k < b1; k++) maxb = std::max(maxb, (fr[k].s + SB - 1) / SB);
     for (int step = 0; step < maxb; step++) {
@@ -380,19 +459,19 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
     }
     int maxrank = -1;
     for (int k = b0; k < b1; k++)
-      if (fr[k].parent >= 0 && fr[k].m > fr[k].s) maxrank = std::max(maxrank, P->sib_rank[P->order[k]]);
+      if (fr[k].parent >= 0 && fr[k].m > fr[k].s) maxrank = std::max(maxrank, sibr[k]);
     for (int rk = 0; rk <= maxrank; rk++) {
       size_t t0 = svt.size();
       for (int k = b0; k < b1; k++) {
         int u = fr[k].m - fr[k].s;
-        if (fr[k].parent < 0 || u == 0 || P->sib_rank[P->order[k]] != rk) continue;
+        if (fr[k].parent < 0 || u == 0 || sibr[k] != rk) continue;
         for (int r0 = 0; r0 < u; r0 += 256) svt.push_back({k, 0, 0, r0, std::min(256, u - r0)});
       }
       push_tasks(SV_EXT, t0, l);
     }
   }
-  for (int l = P->n_levels - 1; l >= 0; l--) {
-    int b0 = P->level_ptr[l], b1 = P->level_ptr[l + 1];
+  for (int l = nlev - 1; l >= 0; l--) {
+    int b0 = lvl[l], b1 = lvl[l + 1];
     size_t t0 = svt.size();
     for (int k = b0; k < b1; k++) {
       int u = fr[k].m - fr[k].s;
@@ -432,27 +511,79 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
     }
   }
 
-  D->ncol = S->n;
-  D->nnz = P->a_nnz;
-  std::vector<long long> adest(P->a_dest, P->a_dest + P->a_nnz);
-  std::vector<int> rel(P->rel, P->rel + (P->rel_ptr[P->n_sn] > 0 ? P->rel_ptr[P->n_sn] : 0));
+  // concatenated A pattern: column j of system q is global column xoff_q + j
+  long long ncol = 0;
+  for (const SysView& v : sys) ncol = std::max(ncol, v.xoff + (long long)v.S->n);
+  std::vector<int> colptr(ncol + 1, 0);
+  {
+    long long base = 0;
+    for (const SysView& v : sys) {
+      for (int j = 0; j <= v.S->n; j++) colptr[v.xoff + j] = (int)(base + v.P->a_colptr[j]);
+      base += v.P->a_nnz;
+    }
+  }
+  D->ncol = (int)ncol;
   if (upload(&D->d_fronts, fr, D->stream) || upload(&D->d_rows, rows, D->stream)
       || upload(&D->d_rel, rel, D->stream) || upload(&D->d_adest, adest, D->stream)
       || upload(&D->d_small, small, D->stream) || upload(&D->d_potrf, potrf, D->stream)
       || upload(&D->d_trsm, trsm, D->stream) || upload(&D->d_gemm, gemm, D->stream)
       || upload(&D->d_ea, ea, D->stream) || upload(&D->d_sv_tasks, svt, D->stream)
+      || upload(&D->d_colptr, colptr, D->stream)
       || dev_malloc((void**)&D->d_info, sizeof(int))
-      || dev_malloc((void**)&D->d_colptr, sizeof(int) * (S->n + 1))
-      || dev_malloc((void**)&D->d_avals, sizeof(double) * (P->a_nnz > 0 ? P->a_nnz : 1))) {
+      || dev_malloc((void**)&D->d_avals, sizeof(double) * (nnz_tot > 0 ? nnz_tot : 1)))
+    return -1;
+  if (hipStreamSynchronize(D->stream) != hipSuccess) return -1;
+  return 0;
+}
+
+// Device schedule of `count` independent systems sharing the launches of
+// every level; the systems' vectors are concatenated at offsets xoff[q].
+extern "C" int tamd_dev_batch_setup(tamd_factor* FB, int count, tamd_factor* const* F,
+                                    const long long* xoff)
+{
+  if (!tamd_dev_available()) {
+    taucs_printf((char*)"taucs_amd: no GPU device available; the numeric phase runs only on the GPU\n");
+    return -1;
+  }
+  tamd_dev* D = new tamd_dev();
+  FB->dev = D;
+  if (hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking) != hipSuccess) {
+    tamd_dev_free(FB);
+    return -1;
+  }
+  std::vector<SysView> v(count);
+  for (int q = 0; q < count; q++) {
+    v[q].S = &F[q]->sym;
+    v[q].P = &F[q]->plan;
+    v[q].xoff = xoff[q];
+  }
+  if (dev_build(D, v) != 0) {
+    tamd_dev_free(FB);
+    return -1;
+  }
+  return 0;
+}
+
+extern "C" int tamd_dev_setup(tamd_factor* F)
+{
+  if (!tamd_dev_available()) {
+    taucs_printf((char*)"taucs_amd: no GPU device available; the numeric phase runs only on the GPU\n");
+    return -1;
+  }
+  tamd_dev* D = new tamd_dev();
+  F->dev = D;
+  if (hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking) != hipSuccess) {
     tamd_dev_free(F);
     return -1;
   }
-  if (hipStreamSynchronize(D->stream) != hipSuccess) {
+  std::vector<SysView> one(1);
+  one[0].S = &F->sym;
+  one[0].P = &F->plan;
+  one[0].xoff = 0;
+  if (dev_build(D, one) != 0) {
     tamd_dev_free(F);
     return -1;
   }
-  D->L_elems = P->L_elems;
-  D->U_elems = P->U_elems;
   return 0;
 }
 
@@ -466,21 +597,16 @@ static inline unsigned grid_for(long long n, int block)
 }
 
 template <typename T>
-static int run_numeric(tamd_dev* D, bool bf16upd)
+static int launch_numeric(tamd_dev* D, bool bf16upd, bool prof)
 {
   hipStream_t st = D->stream;
   T* L = (T*)D->L;
   T* U = (T*)D->U;
   T* Iv = (T*)D->Inv;
-  hipEvent_t e0 = nullptr, e1 = nullptr, a0 = nullptr, a1 = nullptr;
-  const bool prof = g_profiling != 0;
-  memset(&D->timing, 0, sizeof(D->timing));
+  hipEvent_t a0 = nullptr, a1 = nullptr;
   if (prof) {
-    HIPCHECK(hipEventCreate(&e0));
-    HIPCHECK(hipEventCreate(&e1));
     HIPCHECK(hipEventCreate(&a0));
     HIPCHECK(hipEventCreate(&a1));
-    HIPCHECK(hipEventRecord(e0, st));
   }
   hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(D->L_elems, 256)), dim3(256), 0, st, L, D->L_elems);
   if (D->U_elems > 0)
@@ -533,6 +659,51 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
       }
     }
   }
+  if (prof) {
+    (void)hipEventDestroy(a0);
+    (void)hipEventDestroy(a1);
+  }
+  D->timing.launches = launches;
+  return 0;
+}
+
+// Runs the whole numeric schedule.  Without profiling the launch sequence is
+// captured once into a hipGraph (kernel nodes only) and replayed afterwards.
+template <typename T>
+static int run_numeric(tamd_dev* D, bool bf16upd)
+{
+  hipStream_t st = D->stream;
+  const bool prof = g_profiling != 0;
+  memset(&D->timing, 0, sizeof(D->timing));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (prof) {
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, st));
+  }
+  const int key = (int)sizeof(T) * 2 + (bf16upd ? 1 : 0);
+  if (prof || !g_graphs) {
+    if (launch_numeric<T>(D, bf16upd, prof) != 0) return -1;
+  } else {
+    if (!D->num_graph || D->num_graph_key != key) {
+      if (D->num_graph) (void)hipGraphExecDestroy(D->num_graph);
+      D->num_graph = nullptr;
+      hipGraph_t g = nullptr;
+      HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = launch_numeric<T>(D, bf16upd, false);
+      HIPCHECK(hipStreamEndCapture(st, &g));
+      if (rc != 0) {
+        (void)hipGraphDestroy(g);
+        return -1;
+      }
+      hipError_t ie = hipGraphInstantiate(&D->num_graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHECK(ie);
+      D->num_graph_key = key;
+    }
+    HIPCHECK(hipGraphLaunch(D->num_graph, st));
+    D->timing.launches = (int)D->ops.size() + 3;
+  }
   HIPCHECK(hipGetLastError());
   if (prof) {
     float ms = 0;
@@ -540,10 +711,9 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
     HIPCHECK(hipEventSynchronize(e1));
     HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
     D->timing.total_ms = ms;
-    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(a0); (void)hipEventDestroy(a1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   }
-  D->timing.launches = launches;
   D->timing.gemm_flops = D->gemm_flops;
   D->timing.assembly_bytes = D->assembly_bytes;
   int info = 0;
@@ -556,10 +726,9 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
   return 0;
 }
 
-extern "C" int tamd_dev_numeric(tamd_factor* F, const taucs_ccs_matrix* A)
+static int ensure_numeric_buffers(tamd_factor* F)
 {
   tamd_dev* D = F->dev;
-  if (!D) return -1;
   int prec = F->precision;
   int elem = prec == TAMD_PREC_FP64 ? 8 : 4;
   if (D->prec != prec) {
@@ -573,14 +742,24 @@ extern "C" int tamd_dev_numeric(tamd_factor* F, const taucs_ccs_matrix* A)
     D->prec = prec;
     D->elem = elem;
   }
+  return 0;
+}
+
+// Numeric factorization from the concatenated A values (host pointer).
+extern "C" int tamd_dev_numeric_values(tamd_factor* F, const double* vals)
+{
+  tamd_dev* D = F->dev;
+  if (!D || ensure_numeric_buffers(F) != 0) return -1;
   tamd_refine_forget(F);
-  HIPCHECK(hipMemcpyAsync(D->d_colptr, A->colptr, sizeof(int) * (A->n + 1), hipMemcpyHostToDevice, D->stream));
-  HIPCHECK(hipMemcpyAsync(D->d_avals, A->values.d, sizeof(double) * (size_t)F->plan.a_nnz,
-                          hipMemcpyHostToDevice, D->stream));
-  int rc;
-  if (prec == TAMD_PREC_FP64) rc = run_numeric<double>(D, false);
-  else rc = run_numeric<float>(D, prec == TAMD_PREC_BF16UPD);
-  return rc;
+  HIPCHECK(hipMemcpyAsync(D->d_avals, vals, sizeof(double) * (size_t)D->nnz, hipMemcpyHostToDevice,
+                          D->stream));
+  if (F->precision == TAMD_PREC_FP64) return run_numeric<double>(D, false);
+  return run_numeric<float>(D, F->precision == TAMD_PREC_BF16UPD);
+}
+
+extern "C" int tamd_dev_numeric(tamd_factor* F, const taucs_ccs_matrix* A)
+{
+  return tamd_dev_numeric_values(F, A->values.d);
 }
 
 extern "C" int taucs_amd_get_numeric_timing(void* vL, taucs_amd_numeric_timing* t)
@@ -606,11 +785,29 @@ static int run_solve(tamd_factor* F, int nrhs, double* dX, int ldx, const double
     int cap = std::max(nrhs, 16);
     free_ptr(D->Part);
     D->Part = nullptr;
+    drop_solve_graphs(D);
     if (dev_malloc(&D->W, sizeof(T) * (size_t)D->W_per_rhs * cap) != 0
         || dev_malloc(&D->Y, sizeof(T) * (size_t)n * cap) != 0
         || dev_malloc(&D->Part, sizeof(T) * 64 * 16 * (size_t)std::max(D->part_slots, 1LL)) != 0)
       return -1;
     D->W_rhs = cap;
+  }
+  const bool use_graph = g_graphs != 0;
+  tamd_dev::SolveGraph* slot = nullptr;
+  if (use_graph) {
+    for (auto& g : D->sv)
+      if (g.exec && g.nrhs == nrhs && g.prec == D->prec && g.x == dX && g.b == dB && g.ldx == ldx
+          && g.ldb == ldb) {
+        g.used = ++D->sv_clock;
+        HIPCHECK(hipGraphLaunch(g.exec, st));
+        return 0;
+      }
+    slot = &D->sv[0];
+    for (auto& g : D->sv)
+      if (g.used < slot->used) slot = &g;
+    if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+    *slot = tamd_dev::SolveGraph();
+    HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   }
   T* W = (T*)D->W;
   T* Y = (T*)D->Y;
@@ -657,6 +854,21 @@ static int run_solve(tamd_factor* F, int nrhs, double* dX, int ldx, const double
                            L, Iv, Y, n, dX, ldx, nrhs);
         break;
     }
+  }
+  if (use_graph) {
+    hipGraph_t g = nullptr;
+    HIPCHECK(hipStreamEndCapture(st, &g));
+    hipError_t ie = hipGraphInstantiate(&slot->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHECK(ie);
+    slot->nrhs = nrhs;
+    slot->prec = D->prec;
+    slot->x = dX;
+    slot->b = dB;
+    slot->ldx = ldx;
+    slot->ldb = ldb;
+    slot->used = ++D->sv_clock;
+    HIPCHECK(hipGraphLaunch(slot->exec, st));
   }
   HIPCHECK(hipGetLastError());
   return 0;

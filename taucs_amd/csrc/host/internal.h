@@ -67,6 +67,7 @@ typedef struct {
   /* A-assembly map: PAPT entry k goes to L[a_dest[k]] */
   int      a_nnz;
   int64_t* a_dest;
+  int*     a_colptr;      /* [n+1] column pointers of the pattern the map was built for */
   uint64_t a_pattern_hash;
 } tamd_plan;
 

@@ -420,6 +420,7 @@ void tamd_plan_free(tamd_plan* P)
   tamd_free(P->rel);
   tamd_free(P->sib_rank);
   tamd_free(P->a_dest);
+  tamd_free(P->a_colptr);
   memset(P, 0, sizeof(*P));
 }
 
@@ -428,9 +429,12 @@ int tamd_plan_amap(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_plan*
   int n = S->n, k, sn, jj, ip;
   int* pos = (int*)tamd_malloc(sizeof(int) * (size_t)(n + 1));
   tamd_free(P->a_dest);
+  tamd_free(P->a_colptr);
   P->a_nnz = A->colptr[n];
   P->a_dest = (int64_t*)tamd_malloc(sizeof(int64_t) * (size_t)(P->a_nnz + 1));
-  if (!pos || !P->a_dest) { tamd_free(pos); return -1; }
+  P->a_colptr = (int*)tamd_malloc(sizeof(int) * (size_t)(n + 1));
+  if (!pos || !P->a_dest || !P->a_colptr) { tamd_free(pos); return -1; }
+  memcpy(P->a_colptr, A->colptr, sizeof(int) * (size_t)(n + 1));
   for (ip = 0; ip < P->a_nnz; ip++) P->a_dest[ip] = -1;
   for (k = 0; k < P->n_live; k++) {
     int m;

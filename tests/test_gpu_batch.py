@@ -1,0 +1,96 @@
+"""Batched factorization (many independent systems in shared level launches)
+and hipGraph replay, against single-system runs."""
+import numpy as np
+import pytest
+
+import taucs_amd as T
+from tests.golden.make_golden import KEYS
+
+pytestmark = pytest.mark.gpu
+
+
+def shifted_systems(count, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(lo, hi + 1, count)
+    mats = []
+    for q, s in enumerate(sides):
+        A = T.mesh2d_shifted(int(s), seed=1000 + q)
+        perm, inv = T.order(A, "genmmd")
+        mats.append(T.permute(A, perm, inv))
+    return mats
+
+
+def test_batch_equals_alone_bitwise():
+    print("case: batch of 24 shifted 2D systems vs one at a time", flush=True)
+    mats = shifted_systems(24, 16, 64, 7)
+    B = T.Batch(mats)
+    b = np.concatenate([T.random_uniform(m.n, 50 + q) for q, m in enumerate(mats)])
+    x = B.solve(b)
+    for q, m in enumerate(mats):
+        F = T.factor_llt_mf(m)
+        xq = F.solve(b[B.xoff[q]:B.xoff[q + 1]])
+        assert np.array_equal(x[B.xoff[q]:B.xoff[q + 1]], xq), q
+        S1, S2 = B.system_supernodes(q), F.supernodes()
+        for k in KEYS:
+            assert np.array_equal(S1[k], S2[k])
+
+
+def test_batch_fp32_refined():
+    print("case: batch fp32 + refinement", flush=True)
+    mats = shifted_systems(16, 16, 48, 3)
+    B = T.Batch(mats, precision=T.FP32)
+    b = np.concatenate([T.random_uniform(m.n, q) for q, m in enumerate(mats)])
+    x, info = B.solve_refined(b, method=0, tol=1e-14, maxit=50)
+    assert info.converged and info.backward_error <= 1e-14
+    for q, m in enumerate(mats):
+        xs, bs = x[B.xoff[q]:B.xoff[q + 1]], b[B.xoff[q]:B.xoff[q + 1]]
+        M = m.to_scipy_full()
+        r = bs - M @ xs
+        eta = abs(r).max() / (abs(M).sum(1).max() * abs(xs).max() + abs(bs).max())
+        assert eta <= 1e-13, q
+
+
+def test_batch_refactor_new_values():
+    print("case: batch numeric refactorization", flush=True)
+    mats = shifted_systems(8, 20, 40, 11)
+    B = T.Batch(mats)
+    b = np.concatenate([T.random_uniform(m.n, q) for q, m in enumerate(mats)])
+    x1 = B.solve(b)
+    scaled = []
+    for m in mats:
+        n, cp, ri, v = m.arrays()
+        scaled.append(T.from_arrays(n, cp, ri, 4.0 * v))
+    assert B.numeric(scaled) == 0
+    x2 = B.solve(b)
+    assert np.allclose(x2, 0.25 * x1, rtol=1e-12, atol=1e-15)
+
+
+def test_graph_replays_match_eager():
+    """Captured numeric and solve graphs, replayed several times with eager
+    work in between, give exactly the eager results."""
+    print("case: hipGraph replay vs eager", flush=True)
+    A = T.mesh3d(18)
+    perm, inv = T.order(A, "genmmd")
+    PA = T.permute(A, perm, inv)
+    b = np.stack([T.random_uniform(A.n, 9), T.random_uniform(A.n, 10)], 1)
+    T.lib.taucs_amd_set_graphs(0)
+    try:
+        Fe = T.factor_llt_mf(PA)
+        xe = Fe.solve(b)
+        Le = Fe.diag()
+    finally:
+        T.lib.taucs_amd_set_graphs(1)
+    F = T.factor_symbolic(PA)
+    for rep in range(4):
+        assert F.numeric(PA) == 0
+        assert np.array_equal(F.diag(), Le)
+        for _ in range(2):
+            assert np.array_equal(F.solve(b), xe)
+        # eager work in between the replays
+        T.lib.taucs_amd_set_graphs(0)
+        try:
+            G = T.factor_llt_mf(PA)
+            assert np.array_equal(G.solve(b), xe)
+            G.free()
+        finally:
+            T.lib.taucs_amd_set_graphs(1)
