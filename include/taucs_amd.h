@@ -115,6 +115,18 @@ int   taucs_amd_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int   taucs_amd_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int   taucs_amd_device_synchronize(void);
 
+/* ---- device memory accounting (admission by HBM budget) ----------------- */
+/* Bytes of device memory the factor L (symbolic or numeric) holds once it is
+   factored in `precision` and has solved up to `nrhs` right-hand sides from
+   host buffers; `refine` adds the FP64 refinement context.  Computed on the
+   host from the schedule, without a device. */
+long long taucs_amd_factor_device_bytes(void* L, int precision, int nrhs, int refine);
+/* Live bytes of all device allocations this library holds right now. */
+long long taucs_amd_device_bytes_in_use(void);
+/* The library is thread-safe across distinct factors: each factor runs on its
+   own HIP stream, so calls on different factors from different threads
+   overlap on the device.  Calls on one factor must not overlap. */
+
 /* ---- timing of the last numeric factorization (device events) ----------- */
 typedef struct {
   double total_ms;       /* numeric factorization, stream time */

@@ -112,6 +112,8 @@ _sigs = {
     "taucs_amd_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "taucs_amd_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "taucs_amd_device_synchronize": (C.c_int, []),
+    "taucs_amd_factor_device_bytes": (C.c_longlong, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "taucs_amd_device_bytes_in_use": (C.c_longlong, []),
     "taucs_amd_get_numeric_timing": (C.c_int, [C.c_void_p, P(NumericTiming)]),
 }
 for _name, (_res, _args) in _sigs.items():
@@ -297,6 +299,10 @@ class Factor:
         if rc != 0:
             raise RuntimeError("refined solve failed")
         return (X[:, 0].copy() if one else X), info
+
+    def device_bytes(self, precision=FP64, nrhs=1, refine=False):
+        """Device bytes this factor holds once factored and solved (host estimate)."""
+        return lib.taucs_amd_factor_device_bytes(self.h, precision, nrhs, int(bool(refine)))
 
     def solve_device(self, dX, dB, n, nrhs):
         """Solve with right-hand sides and solutions already in HBM."""

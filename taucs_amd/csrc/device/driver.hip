@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "kernels.h"
@@ -76,6 +78,7 @@ struct tamd_dev {
   std::vector<int> lvl_ptr;      // fronts of level l: [lvl_ptr[l], lvl_ptr[l+1])
   int n_fronts = 0, ncol = 0, nnz = 0, n_levels = 0;
   long long W_per_rhs = 0;
+  size_t struct_bytes = 0;       // device bytes of the schedule and A values
   double gemm_flops = 0, assembly_bytes = 0;
   int gemm_launches = 0;
   // numeric arenas (precision-dependent)
@@ -129,6 +132,15 @@ static void drop_graphs(tamd_dev* D)
   drop_solve_graphs(D);
 }
 
+// Every device allocation of the library goes through dev_malloc/free_ptr,
+// which keep a ledger of live bytes (for admission by HBM budget) under one
+// mutex.  The same mutex is held from stream-capture begin to graph
+// instantiation, so no thread frees or allocates device memory (which may
+// synchronize the device) while another thread's stream is being captured.
+static std::mutex g_mem_mu;
+static std::unordered_map<void*, size_t> g_live;
+static size_t g_live_bytes = 0;
+
 static int dev_malloc(void** p, size_t bytes)
 {
   if (tamd_alloc_should_fail()) {
@@ -136,13 +148,38 @@ static int dev_malloc(void** p, size_t bytes)
     taucs_printf((char*)"taucs_amd: device allocation failed (induced)\n");
     return -1;
   }
-  hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+  const size_t b = bytes ? bytes : 1;
+  std::lock_guard<std::mutex> lk(g_mem_mu);
+  hipError_t e = hipMalloc(p, b);
   if (e != hipSuccess) {
     *p = nullptr;
     taucs_printf((char*)"taucs_amd: hipMalloc(%zu) failed: %s\n", bytes, hipGetErrorString(e));
     return -1;
   }
+  g_live[*p] = b;
+  g_live_bytes += b;
   return 0;
+}
+
+static void free_ptr(void* p)
+{
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(g_mem_mu);
+  (void)hipFree(p);
+  auto it = g_live.find(p);
+  if (it != g_live.end()) {
+    g_live_bytes -= it->second;
+    g_live.erase(it);
+  }
+}
+
+extern "C" int tamd_dev_malloc(void** p, size_t bytes) { return dev_malloc(p, bytes); }
+extern "C" void tamd_dev_free_ptr(void* p) { free_ptr(p); }
+
+extern "C" long long taucs_amd_device_bytes_in_use(void)
+{
+  std::lock_guard<std::mutex> lk(g_mem_mu);
+  return (long long)g_live_bytes;
 }
 
 template <typename V>
@@ -158,11 +195,6 @@ extern "C" int tamd_dev_available(void)
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n > 0;
-}
-
-static void free_ptr(void* p)
-{
-  if (p) (void)hipFree(p);
 }
 
 extern "C" void tamd_dev_free_numeric(tamd_factor* F)
@@ -221,7 +253,7 @@ struct SysView {
 // Builds the device schedule of one or more independent systems.  Fronts of
 // level l of every system share the launches of level l; each system keeps
 // its own arenas (offsets), row numbering (shifted by xoff) and tree.
-static int dev_build(tamd_dev* D, const std::vector<SysView>& sys)
+static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = false)
 {
   int nlev = 0, nf = 0;
   long long nnz_tot = 0;
@@ -523,6 +555,13 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys)
     }
   }
   D->ncol = (int)ncol;
+  D->struct_bytes = sizeof(FrontDesc) * fr.size() + sizeof(int) * (rows.size() + rel.size())
+                    + sizeof(long long) * adest.size()
+                    + sizeof(int) * (small.size() + colptr.size() + 1)
+                    + sizeof(PanelTask) * (potrf.size() + svt.size())
+                    + sizeof(GemmTask) * (trsm.size() + gemm.size()) + sizeof(EATask) * ea.size()
+                    + sizeof(double) * (size_t)nnz_tot;
+  if (dry) return 0;
   if (upload(&D->d_fronts, fr, D->stream) || upload(&D->d_rows, rows, D->stream)
       || upload(&D->d_rel, rel, D->stream) || upload(&D->d_adest, adest, D->stream)
       || upload(&D->d_small, small, D->stream) || upload(&D->d_potrf, potrf, D->stream)
@@ -585,6 +624,34 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
     return -1;
   }
   return 0;
+}
+
+// Device bytes a factor will hold after a numeric factorization in
+// `precision` and solves of up to `nrhs` right-hand sides from host buffers
+// (plus the refinement context when `refine` is set), computed on the host
+// from the same schedule construction, without touching the device.
+extern "C" long long taucs_amd_factor_device_bytes(void* vL, int precision, int nrhs, int refine)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  if (!F || nrhs < 0) return -1;
+  tamd_dev D;
+  std::vector<SysView> one(1);
+  one[0].S = &F->sym;
+  one[0].P = &F->plan;
+  one[0].xoff = 0;
+  if (dev_build(&D, one, true) != 0) return -1;
+  const long long elem = precision == TAMD_PREC_FP64 ? 8 : 4;
+  const long long n = F->n, cap = std::max(nrhs, 16);
+  long long b = (long long)D.struct_bytes + sizeof(int);
+  b += elem * (std::max(D.L_elems, 1LL) + std::max(D.U_elems, 1LL) + std::max(D.I_elems, 1LL));
+  b += elem * (D.W_per_rhs * cap + n * cap + 64LL * 16 * std::max(D.part_slots, 1LL));
+  b += 2LL * 8 * n * std::max(nrhs, 1);
+  if (refine) {
+    const long long full = 2LL * F->plan.a_nnz;
+    b += 4 * (n + 1) + 12 * std::max(full, 1LL);
+    b += 6LL * 8 * n * std::max(nrhs, 1) + 8LL * (256 + 8) * std::max(nrhs, 1);
+  }
+  return b;
 }
 
 // ---- numeric factorization ------------------------------------------------
@@ -689,6 +756,7 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
       if (D->num_graph) (void)hipGraphExecDestroy(D->num_graph);
       D->num_graph = nullptr;
       hipGraph_t g = nullptr;
+      std::lock_guard<std::mutex> cap_lk(g_mem_mu);
       HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       int rc = launch_numeric<T>(D, bf16upd, false);
       HIPCHECK(hipStreamEndCapture(st, &g));
@@ -794,6 +862,7 @@ static int run_solve(tamd_factor* F, int nrhs, double* dX, int ldx, const double
   }
   const bool use_graph = g_graphs != 0;
   tamd_dev::SolveGraph* slot = nullptr;
+  std::unique_lock<std::mutex> cap_lk(g_mem_mu, std::defer_lock);
   if (use_graph) {
     for (auto& g : D->sv)
       if (g.exec && g.nrhs == nrhs && g.prec == D->prec && g.x == dX && g.b == dB && g.ldx == ldx
@@ -807,6 +876,7 @@ static int run_solve(tamd_factor* F, int nrhs, double* dX, int ldx, const double
       if (g.used < slot->used) slot = &g;
     if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
     *slot = tamd_dev::SolveGraph();
+    cap_lk.lock();
     HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   }
   T* W = (T*)D->W;

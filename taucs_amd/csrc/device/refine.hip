@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "kernels.h"
@@ -82,14 +83,14 @@ __global__ __launch_bounds__(256) void k_colreduce(int n, const double* __restri
 
 __global__ void k_colfinal(int nblk, int mode, const double* partial, double* out, int nrhs)
 {
-  int r = threadIdx.x;
-  if (r >= nrhs) return;
-  double acc = 0.0;
-  for (int q = 0; q < nblk; q++) {
-    double v = partial[q + (long long)r * nblk];
-    acc = mode == 1 ? fmax(acc, v) : acc + v;
+  for (int r = threadIdx.x; r < nrhs; r += blockDim.x) {
+    double acc = 0.0;
+    for (int q = 0; q < nblk; q++) {
+      double v = partial[q + (long long)r * nblk];
+      acc = mode == 1 ? fmax(acc, v) : acc + v;
+    }
+    out[r] = acc;
   }
-  out[r] = acc;
 }
 
 // y(:,r) += alpha[r] * x(:,r)   (sign: +1 or -1 applied to alpha)
@@ -115,9 +116,8 @@ __global__ void k_xpby_col(int n, const double* __restrict__ z, const double* __
 // out[r] = (den[r] != 0) ? num[r]/den[r] : 0, optionally masked by done[r]
 __global__ void k_ratio(const double* num, const double* den, const int* done, double* out, int nrhs)
 {
-  int r = threadIdx.x;
-  if (r >= nrhs) return;
-  out[r] = (den[r] != 0.0 && !(done && done[r])) ? num[r] / den[r] : 0.0;
+  for (int r = threadIdx.x; r < nrhs; r += blockDim.x)
+    out[r] = (den[r] != 0.0 && !(done && done[r])) ? num[r] / den[r] : 0.0;
 }
 
 }  // namespace tamd
@@ -135,43 +135,54 @@ struct RefineCtx {
   int cap = 0;  // rhs capacity of the work vectors
   double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *x = nullptr, *b = nullptr;
   double *partial = nullptr, *s1 = nullptr, *s2 = nullptr, *s3 = nullptr;
+  double *mask = nullptr, *rz = nullptr, *rz_new = nullptr, *scal = nullptr;
   int* done = nullptr;
 };
 
 struct CtxEntry { void* key; RefineCtx* ctx; };
+// Contexts of all factors; the list is shared by threads, a context is used
+// by one thread at a time (the caller's contract for one factor).
 static std::vector<CtxEntry> g_ctx;
+static std::mutex g_ctx_mu;
+
+extern "C" int tamd_dev_malloc(void** p, size_t bytes);
+extern "C" void tamd_dev_free_ptr(void* p);
 
 static void ctx_free(RefineCtx* c)
 {
   if (!c) return;
-  void* ps[] = {c->rp, c->ci, c->v, c->r, c->z, c->p, c->q, c->x, c->b, c->partial, c->s1, c->s2, c->s3, c->done};
-  for (void* p : ps)
-    if (p) (void)hipFree(p);
+  void* ps[] = {c->rp, c->ci, c->v, c->r, c->z, c->p, c->q, c->x, c->b, c->partial, c->s1, c->s2, c->s3,
+                c->mask, c->rz, c->rz_new, c->scal, c->done};
+  for (void* p : ps) tamd_dev_free_ptr(p);
   delete c;
 }
 
 extern "C" void tamd_refine_forget(void* key)
 {
-  for (size_t i = 0; i < g_ctx.size(); i++)
-    if (g_ctx[i].key == key) {
-      ctx_free(g_ctx[i].ctx);
-      g_ctx.erase(g_ctx.begin() + i);
-      return;
-    }
+  RefineCtx* c = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    for (size_t i = 0; i < g_ctx.size(); i++)
+      if (g_ctx[i].key == key) {
+        c = g_ctx[i].ctx;
+        g_ctx.erase(g_ctx.begin() + i);
+        break;
+      }
+  }
+  ctx_free(c);
 }
 
-static int dmalloc(void** p, size_t bytes)
-{
-  if (tamd_alloc_should_fail()) return -1;
-  return hipMalloc(p, bytes ? bytes : 1) == hipSuccess ? 0 : -1;
-}
+static int dmalloc(void** p, size_t bytes) { return tamd_dev_malloc(p, bytes); }
 
 // Build (or reuse) the full-pattern CSR of A on the device.
 static RefineCtx* get_ctx(void* key, const taucs_ccs_matrix* A, int nrhs)
 {
   RefineCtx* c = nullptr;
-  for (auto& e : g_ctx)
-    if (e.key == key) c = e.ctx;
+  {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    for (auto& e : g_ctx)
+      if (e.key == key) c = e.ctx;
+  }
   if (c && (c->A != A || c->vals_ptr != A->values.d || c->n != A->n || c->nnz != A->colptr[A->n])) {
     tamd_refine_forget(key);
     c = nullptr;
@@ -215,22 +226,26 @@ static RefineCtx* get_ctx(void* key, const taucs_ccs_matrix* A, int nrhs)
       ctx_free(c);
       return nullptr;
     }
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
     g_ctx.push_back({key, c});
   }
   if (c->cap < nrhs) {
     void** vs[] = {(void**)&c->r, (void**)&c->z, (void**)&c->p, (void**)&c->q, (void**)&c->x, (void**)&c->b};
+    c->cap = 0;
     for (void** pp : vs) {
-      if (*pp) (void)hipFree(*pp);
+      tamd_dev_free_ptr(*pp);
       *pp = nullptr;
       if (dmalloc(pp, sizeof(double) * (size_t)c->n * nrhs)) return nullptr;
     }
-    void** ss[] = {(void**)&c->partial, (void**)&c->s1, (void**)&c->s2, (void**)&c->s3, (void**)&c->done};
-    size_t sz[] = {sizeof(double) * 256 * (size_t)nrhs, sizeof(double) * nrhs, sizeof(double) * nrhs,
-                   sizeof(double) * nrhs, sizeof(int) * nrhs};
-    for (int i = 0; i < 5; i++) {
-      if (*ss[i]) (void)hipFree(*ss[i]);
+    void** ss[] = {(void**)&c->partial, (void**)&c->s1,     (void**)&c->s2,   (void**)&c->s3,
+                   (void**)&c->mask,    (void**)&c->rz,     (void**)&c->rz_new, (void**)&c->scal,
+                   (void**)&c->done};
+    for (int i = 0; i < 9; i++) {
+      size_t sz = i == 0 ? sizeof(double) * 256 * (size_t)nrhs
+                         : (i == 8 ? sizeof(int) : sizeof(double)) * (size_t)nrhs;
+      tamd_dev_free_ptr(*ss[i]);
       *ss[i] = nullptr;
-      if (dmalloc(ss[i], sz[i])) return nullptr;
+      if (dmalloc(ss[i], sz)) return nullptr;
     }
     c->cap = nrhs;
   }
@@ -300,18 +315,27 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
   int it = 0;
   bool conv = false;
   const unsigned gv = (unsigned)(((long long)n * nrhs + 255) / 256);
+  // Every right-hand side follows its own recurrence and stops changing once
+  // its backward error reaches tol (its step is masked to zero), so a column's
+  // result does not depend on which other columns are solved with it.
+  std::vector<int> done(nrhs, 0);
+  std::vector<double> mask(nrhs, 1.0);
   if (method == 0) {
     // x = M^-1 b; repeat r = b - A x, x += M^-1 r
     if (tamd_dev_solve_device(F, nrhs, c->x, n, c->b, n)) return -1;
     for (;;) {
       if (residual_and_error(st, c, c->x, c->b, c->r, nrhs, eta, relres)) return -1;
-      double worst = *std::max_element(eta.begin(), eta.end());
-      if (worst <= tol) { conv = true; break; }
+      bool all = true;
+      for (int q = 0; q < nrhs; q++) {
+        if (eta[q] <= tol) done[q] = 1;
+        mask[q] = done[q] ? 0.0 : 1.0;
+        all = all && done[q];
+      }
+      if (all) { conv = true; break; }
       if (it >= maxit) break;
       if (tamd_dev_solve_device(F, nrhs, c->z, n, c->r, n)) return -1;
-      std::vector<double> ones(nrhs, 1.0);
-      HIPOK(hipMemcpyAsync(c->s3, ones.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(k_axpy_col, dim3(gv), dim3(256), 0, st, n, c->s3, 1.0, c->z, c->x, nrhs);
+      HIPOK(hipMemcpyAsync(c->mask, mask.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_axpy_col, dim3(gv), dim3(256), 0, st, n, c->mask, 1.0, c->z, c->x, nrhs);
       it++;
     }
   } else {
@@ -321,28 +345,18 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
     if (tamd_dev_solve_device(F, nrhs, c->z, n, c->r, n)) return -1;
     HIPOK(hipMemcpyAsync(c->p, c->z, sizeof(double) * n * nrhs, hipMemcpyDeviceToDevice, st));
     HIPOK(hipMemsetAsync(c->done, 0, sizeof(int) * nrhs, st));
-    double* rz = nullptr;
-    double* rz_new = nullptr;
-    double* scal = nullptr;
-    if (dmalloc((void**)&rz, sizeof(double) * nrhs) || dmalloc((void**)&rz_new, sizeof(double) * nrhs)
-        || dmalloc((void**)&scal, sizeof(double) * nrhs)) {
-      if (rz) (void)hipFree(rz);
-      if (rz_new) (void)hipFree(rz_new);
-      return -1;
-    }
-    std::vector<int> done(nrhs, 0);
-    colreduce(st, c, c->r, c->z, 0, rz, nrhs);
+    colreduce(st, c, c->r, c->z, 0, c->rz, nrhs);
     for (;;) {
       // q = A p ; alpha = rz / (p.q)
       hipLaunchKernelGGL(k_csr_residual, dim3((n + 255) / 256), dim3(256), 0, st, n, c->rp, c->ci, c->v,
                          c->p, n, (const double*)nullptr, n, c->q, n, nrhs);
       colreduce(st, c, c->p, c->q, 0, c->s1, nrhs);
-      hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, st, rz, c->s1, c->done, scal, nrhs);
-      hipLaunchKernelGGL(k_axpy_col, dim3(gv), dim3(256), 0, st, n, scal, 1.0, c->p, c->x, nrhs);
-      hipLaunchKernelGGL(k_axpy_col, dim3(gv), dim3(256), 0, st, n, scal, -1.0, c->q, c->r, nrhs);
+      hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, st, c->rz, c->s1, c->done, c->scal, nrhs);
+      hipLaunchKernelGGL(k_axpy_col, dim3(gv), dim3(256), 0, st, n, c->scal, 1.0, c->p, c->x, nrhs);
+      hipLaunchKernelGGL(k_axpy_col, dim3(gv), dim3(256), 0, st, n, c->scal, -1.0, c->q, c->r, nrhs);
       it++;
       // convergence on the true residual every iteration (k small)
-      if (residual_and_error(st, c, c->x, c->b, c->q, nrhs, eta, relres)) break;
+      if (residual_and_error(st, c, c->x, c->b, c->q, nrhs, eta, relres)) return -1;
       bool all = true;
       for (int q = 0; q < nrhs; q++) {
         if (eta[q] <= tol) done[q] = 1;
@@ -351,16 +365,14 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
       if (all) { conv = true; break; }
       if (it >= maxit) break;
       HIPOK(hipMemcpyAsync(c->done, done.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
-      if (tamd_dev_solve_device(F, nrhs, c->z, n, c->r, n)) break;
-      colreduce(st, c, c->r, c->z, 0, rz_new, nrhs);
-      hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, st, rz_new, rz, (const int*)nullptr, scal, nrhs);
-      hipLaunchKernelGGL(k_xpby_col, dim3(gv), dim3(256), 0, st, n, c->z, scal, c->p, nrhs);
-      HIPOK(hipMemcpyAsync(rz, rz_new, sizeof(double) * nrhs, hipMemcpyDeviceToDevice, st));
+      if (tamd_dev_solve_device(F, nrhs, c->z, n, c->r, n)) return -1;
+      colreduce(st, c, c->r, c->z, 0, c->rz_new, nrhs);
+      hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, st, c->rz_new, c->rz, (const int*)nullptr, c->scal,
+                         nrhs);
+      hipLaunchKernelGGL(k_xpby_col, dim3(gv), dim3(256), 0, st, n, c->z, c->scal, c->p, nrhs);
+      HIPOK(hipMemcpyAsync(c->rz, c->rz_new, sizeof(double) * nrhs, hipMemcpyDeviceToDevice, st));
     }
-    (void)hipFree(rz);
-    (void)hipFree(rz_new);
-    (void)hipFree(scal);
-    residual_and_error(st, c, c->x, c->b, c->r, nrhs, eta, relres);
+    if (residual_and_error(st, c, c->x, c->b, c->r, nrhs, eta, relres)) return -1;
   }
   HIPOK(hipMemcpy2DAsync(X, sizeof(double) * ldx, c->x, sizeof(double) * n, sizeof(double) * n, nrhs,
                          hipMemcpyDeviceToHost, st));
