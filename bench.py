@@ -61,6 +61,168 @@ def cpu_baseline(sample_side=30, nrhs=8):
                       f"{nrhs}-rhs solve {ts:.2f} s ({nrhs / ts:.2f} solves/s)"}
 
 
+def host_backward_error(M, X, B):
+    """Max over columns of |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf), FP64 on the host."""
+    X = X.reshape(B.shape)
+    R = B - M @ X
+    anorm = abs(M).sum(1).max()
+    return float((abs(R).max(0) / (anorm * abs(X).max(0) + abs(B).max(0))).max())
+
+
+def bench_varcoef(T, side=80, nrhs=4, steps=2, seed=2024):
+    """Variable-coefficient 3D operator (log-uniform conductivities over
+    [1e-2, 1e2], shift 1e-6): plain FP64 factor + solve against FP32 fronts +
+    FP64 refinement (classical and CG) and the BF16-update variant, each to
+    backward error <= 1e-12, with the forward error against the FP64 result."""
+    t0 = time.perf_counter()
+    A = T.mesh3d_varcoef(side, side, side, seed)
+    perm, inv = T.order(A, "genmmd")
+    PA = T.permute(A, perm, inv)
+    F = T.factor_symbolic(PA)
+    fi = F.info()
+    n = PA.n
+    log(f"varcoef {side}^3: n {n}, symbolic {time.perf_counter() - t0:.1f} s, "
+        f"fund flops {fi.fund_flops:.3e}")
+    B = np.stack([T.random_uniform(n, 100 + r) for r in range(nrhs)], 1)
+    M = PA.to_scipy_full()
+    out = {"workload": f"mesh3d_varcoef {side}^3 (n={n}), conductivities log-uniform [1e-2,1e2], "
+                       f"shift 1e-6, genmmd, {nrhs} rhs", "fund_flops": fi.fund_flops}
+
+    def run(prec, method, tol, maxit):
+        tf, ts, its, bws, ok = [], [], [], [], True
+        X = None
+        for i in range(steps + 1):
+            t_a = time.perf_counter()
+            if F.numeric(PA, prec) != 0:
+                return None
+            t_b = time.perf_counter()
+            if method is None:
+                X = F.solve(B)
+                info = None
+            else:
+                X, info = F.solve_refined(PA, B, method=method, tol=tol, maxit=maxit)
+            t_c = time.perf_counter()
+            if i == 0:
+                continue          # warm-up (graph capture, buffers)
+            tf.append(t_b - t_a)
+            ts.append(t_c - t_b)
+            if info is not None:
+                its.append(info.iterations)
+                ok = ok and bool(info.converged)
+        bwd = host_backward_error(M, X, B)
+        return {"factor_ms": round(1e3 * float(np.mean(tf)), 2),
+                "solve_ms": round(1e3 * float(np.mean(ts)), 2),
+                "factor_gflops": round(fi.fund_flops / float(np.mean(tf)) / 1e9, 1),
+                "iterations": max(its) if its else 0, "converged": ok,
+                "backward_error": bwd, "X": X}
+
+    r64 = run(T.FP64, None, 0, 0)
+    out["fp64"] = r64
+    x64 = r64["X"]
+    base = r64["factor_ms"] + r64["solve_ms"]
+    for name, prec, method in (("fp32_ir", T.FP32, 0), ("fp32_pcg", T.FP32, 1),
+                               ("bf16upd_ir", T.BF16UPD, 0), ("bf16upd_pcg", T.BF16UPD, 1)):
+        r = run(prec, method, 1e-12, 200)
+        if r is None:
+            out[name] = {"failed": "factorization broke down (pivot not positive)"}
+            log(f"varcoef {name}: breakdown")
+            continue
+        r["forward_error_vs_fp64"] = float(np.linalg.norm(r["X"] - x64) / np.linalg.norm(x64))
+        r["speedup_factor_plus_solve_vs_fp64"] = round(base / (r["factor_ms"] + r["solve_ms"]), 3)
+        out[name] = r
+        log(f"varcoef {name}: {({k: v for k, v in r.items() if k != 'X'})}")
+    for v in out.values():
+        if isinstance(v, dict):
+            v.pop("X", None)
+    log(f"varcoef fp64: {out['fp64']}")
+    F.free()
+    return out
+
+
+def bench_batch(T, count=512, lo=16, hi=128, steps=3, seed=7):
+    """512 independent shifted 2D Laplacians (side uniform on [16, 128],
+    diagonal shift uniform[0,1]) in the same level launches: FP64 and FP32 +
+    refinement, one uniform[-1,1] right-hand side per system."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(lo, hi + 1, count)
+    t0 = time.perf_counter()
+    mats = []
+    for q, s in enumerate(sides):
+        A = T.mesh2d_shifted(int(s), seed=10_000 + q)
+        perm, inv = T.order(A, "genmmd")
+        mats.append(T.permute(A, perm, inv))
+    t_order = time.perf_counter() - t0
+    out = {"workload": f"{count} shifted 2D 5-point Laplacians, side U[{lo},{hi}], genmmd, 1 rhs each"}
+    res = {}
+    Mbd = None
+    for name, prec in (("fp64", T.FP64), ("fp32_ir", T.FP32)):
+        t0 = time.perf_counter()
+        Bt = T.Batch(mats, precision=prec)
+        t_setup = time.perf_counter() - t0
+        n = Bt.n_total
+        b = np.concatenate([T.random_uniform(m.n, 500 + q) for q, m in enumerate(mats)])
+        if Mbd is None:
+            Mbd = sp.block_diag([m.to_scipy_full() for m in mats], format="csr")
+        dB = T.DeviceArray(b.reshape(n, 1))
+        dX = T.DeviceArray(np.zeros((n, 1)))
+        tf, ts = [], []
+        for i in range(steps + 1):
+            t_a = time.perf_counter()
+            if Bt.numeric(None, prec) != 0:
+                raise RuntimeError("batched factorization failed")
+            t_b = time.perf_counter()
+            if prec == T.FP64:
+                Bt.solve_device(dX.ptr, dB.ptr, 1)
+                x = None
+            else:
+                x, info = Bt.solve_refined(b, method=0, tol=1e-13, maxit=50)
+            t_c = time.perf_counter()
+            if i:
+                tf.append(t_b - t_a)
+                ts.append(t_c - t_b)
+        if x is None:
+            x = dX.get()[:, 0]
+        bw = []
+        for q in range(count):
+            sl = slice(int(Bt.xoff[q]), int(Bt.xoff[q + 1]))
+            Mq = Mbd[sl, sl]
+            bw.append(host_backward_error(Mq, x[sl].reshape(-1, 1), b[sl].reshape(-1, 1)))
+        r = {"numeric_ms": round(1e3 * float(np.mean(tf)), 3),
+             "solve_ms": round(1e3 * float(np.mean(ts)), 3),
+             "factor_gflops": round(Bt.fund_flops / float(np.mean(tf)) / 1e9, 2),
+             "systems_per_s": round(count / (float(np.mean(tf)) + float(np.mean(ts))), 1),
+             "max_system_backward_error": float(max(bw)),
+             "setup_s": round(t_setup, 2), "levels": Bt.n_levels}
+        if prec != T.FP64:
+            r["refine_iterations"] = int(info.iterations)
+        res[name] = r
+        log(f"batch {name}: {r}")
+        dB.free()
+        dX.free()
+        Bt.free()
+    out.update(res)
+    out["n_total"] = int(sum(m.n for m in mats))
+    out["ordering_s"] = round(t_order, 2)
+    return out
+
+
+def bench_serving(T, requests=300, rate=40.0, slots=4, budget_gib=16.0, seed=3):
+    """Poisson request mix through the serving loop, FP64 and FP32 + refinement."""
+    from taucs_amd import serving as S
+    out = {}
+    for name, prec in (("fp64", T.FP64), ("fp32_ir", T.FP32)):
+        mix = S.make_mix(requests, rate, seed=seed)
+        srv = S.Server(hbm_budget=int(budget_gib * S.GIB), slots=slots, precision=prec, tol=1e-12)
+        summ = S.run_mix(mix, srv)
+        srv.close()
+        out[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in summ.items()}
+        log(f"serving {name}: {out[name]}")
+    out["workload"] = (f"{requests} requests, Poisson {rate}/s: 70% 2D side U[32,256], 30% 3D side "
+                       f"U[8,40], 1-16 rhs, 40% solve-only; {slots} slots, budget {budget_gib} GiB")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +232,9 @@ def main():
     ap.add_argument("--nrhs", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-varcoef", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
@@ -161,6 +326,14 @@ def main():
         log(f"mixed: {mixed}")
         Fm.free()
 
+    varcoef = batch = serving = None
+    if not args.no_varcoef:
+        varcoef = bench_varcoef(T)
+    if not args.no_batch:
+        batch = bench_batch(T)
+    if not args.no_serving:
+        serving = bench_serving(T)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -200,6 +373,9 @@ def main():
                      "avg_launch_ms": round(gemm_avg_ms, 4), "launches": tm.gemm_launches,
                      "traffic": None},
         "cpu_baseline": cpu,
+        "mixed_precision_varcoef": varcoef,
+        "batched": batch,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

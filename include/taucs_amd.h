@@ -55,6 +55,12 @@ int taucs_amd_factor_get_info(void* L, taucs_amd_factor_info* info);
 int taucs_amd_factor_get_supernodes(void* L, int* sn_size, int* sn_up_size,
                                     int* first_child, int* next_child,
                                     int* struct_ptr, int* struct_idx);
+/* Level schedule of the factor, per supernode (-1 for supernodes merged away):
+   its level (children before parents), its parent (n_sn = the virtual root),
+   and the element offsets of its panel in the L arena and of its update block
+   in the U arena (update blocks of disjoint lifetimes share space). */
+int taucs_amd_factor_get_schedule(void* L, int* sn_level, int* sn_parent, long long* off_L,
+                                  long long* off_U);
 
 /* ---- solves ------------------------------------------------------------- */
 /* X, B column-major host arrays (n x nrhs); permuted space like

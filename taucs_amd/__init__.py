@@ -84,6 +84,7 @@ _sigs = {
     "taucs_allocation_attempts": (C.c_int, []),
     "taucs_amd_factor_get_info": (C.c_int, [C.c_void_p, P(FactorInfo)]),
     "taucs_amd_factor_get_supernodes": (C.c_int, [C.c_void_p] + [C.c_void_p] * 6),
+    "taucs_amd_factor_get_schedule": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
     "taucs_amd_supernodal_solve_many": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                                   C.c_void_p, C.c_int]),
     "taucs_amd_supernodal_solve_many_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int,
@@ -269,6 +270,16 @@ class Factor:
         lib.taucs_amd_factor_get_supernodes(self.h, None, None, None, None, None, _dptr(idx))
         return dict(n_sn=ns, sn_size=sz[:ns], sn_up_size=up[:ns], first_child=fc, next_child=nc,
                     struct_ptr=sp_, struct_idx=idx[: int(sp_[ns])])
+
+    def schedule(self):
+        """Per supernode: level, parent, L offset, U offset (live supernodes only)."""
+        ns = self.info().n_sn
+        lev = np.zeros(ns, dtype=np.int32)
+        par = np.zeros(ns, dtype=np.int32)
+        oL = np.zeros(ns, dtype=np.int64)
+        oU = np.zeros(ns, dtype=np.int64)
+        lib.taucs_amd_factor_get_schedule(self.h, _dptr(lev), _dptr(par), _dptr(oL), _dptr(oU))
+        return {"level": lev, "parent": par, "off_L": oL, "off_U": oU}
 
     def numeric(self, A, precision=None):
         if precision is None:

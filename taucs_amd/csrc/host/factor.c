@@ -336,3 +336,20 @@ int taucs_amd_factor_get_supernodes(void* vL, int* sn_size, int* sn_up_size, int
   if (struct_ptr) struct_ptr[S->n_sn] = p;
   return 0;
 }
+
+int taucs_amd_factor_get_schedule(void* vL, int* sn_level, int* sn_parent, long long* off_L,
+                                  long long* off_U)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  const tamd_plan* P;
+  int sn;
+  if (!F) return -1;
+  P = &F->plan;
+  for (sn = 0; sn < P->n_sn; sn++) {
+    if (sn_level) sn_level[sn] = P->sn_level[sn];
+    if (sn_parent) sn_parent[sn] = P->sn_parent[sn];
+    if (off_L) off_L[sn] = (long long)P->off_L[sn];
+    if (off_U) off_U[sn] = (long long)P->off_U[sn];
+  }
+  return 0;
+}

@@ -457,6 +457,104 @@ int tamd_plan_amap(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_plan*
   return 0;
 }
 
+/* Offsets of the update blocks U_sn (u x u, u = m - s) in one arena whose
+   blocks are reused once dead.  U_sn is written while level(sn) is factored
+   and read by the extend-add into its parent during level(parent); it is dead
+   from level(parent) + 1 on, so its space is handed to fronts of later
+   levels (first fit over a coalescing free list, deterministic).  The
+   reference keeps each update matrix on the heap from its front's return
+   until the parent has assembled it (src/taucs_sn_llt.c:1925-1940); a level
+   schedule gives the same lifetimes per level. */
+typedef struct { int64_t off, len; } uspan;
+
+static int plan_update_blocks(const tamd_symbolic* S, tamd_plan* P, int64_t* total)
+{
+  int nsn = S->n_sn, nl = P->n_levels, k, lev;
+  int64_t top = 0, peak = 0;   /* current end of the used space, and its maximum */
+  int nfree = 0, cap = 64;
+  uspan* fl = (uspan*)tamd_malloc(sizeof(uspan) * (size_t)cap);
+  int* dies = (int*)tamd_malloc(sizeof(int) * (size_t)(P->n_live + 1));   /* fronts by death level */
+  int* dptr = (int*)tamd_calloc((size_t)nl + 2, sizeof(int));
+  if (!fl || !dies || !dptr) { tamd_free(fl); tamd_free(dies); tamd_free(dptr); return -1; }
+  for (k = 0; k < P->n_live; k++) {
+    int sn = P->order[k], p = P->sn_parent[sn];
+    if (p < nsn) dptr[P->sn_level[p] + 1]++;
+  }
+  for (lev = 0; lev <= nl; lev++) dptr[lev + 1] += dptr[lev];
+  {
+    int* fill = (int*)tamd_malloc(sizeof(int) * (size_t)(nl + 1));
+    if (!fill) { tamd_free(fl); tamd_free(dies); tamd_free(dptr); return -1; }
+    memcpy(fill, dptr, sizeof(int) * (size_t)(nl + 1));
+    for (k = 0; k < P->n_live; k++) {
+      int sn = P->order[k], p = P->sn_parent[sn];
+      if (p < nsn) dies[fill[P->sn_level[p]]++] = sn;
+    }
+    tamd_free(fill);
+  }
+  for (lev = 0; lev < nl; lev++) {
+    int e;
+    /* blocks consumed by the extend-adds of level lev-1 are free now */
+    if (lev > 0)
+      for (e = dptr[lev - 1]; e < dptr[lev]; e++) {
+        int sn = dies[e], i, j;
+        int64_t u = (int64_t)(S->sn_up_size[sn] - S->sn_size[sn]);
+        uspan b;
+        if (u == 0) continue;
+        b.off = P->off_U[sn];
+        b.len = u * u;
+        /* insert sorted by offset, then merge with neighbours */
+        if (nfree == cap) {
+          uspan* nf = (uspan*)tamd_realloc(fl, sizeof(uspan) * (size_t)cap * 2);
+          if (!nf) { tamd_free(fl); tamd_free(dies); tamd_free(dptr); return -1; }
+          fl = nf;
+          cap *= 2;
+        }
+        for (i = nfree; i > 0 && fl[i - 1].off > b.off; i--) fl[i] = fl[i - 1];
+        fl[i] = b;
+        nfree++;
+        if (i + 1 < nfree && fl[i].off + fl[i].len == fl[i + 1].off) {
+          fl[i].len += fl[i + 1].len;
+          for (j = i + 1; j + 1 < nfree; j++) fl[j] = fl[j + 1];
+          nfree--;
+        }
+        if (i > 0 && fl[i - 1].off + fl[i - 1].len == fl[i].off) {
+          fl[i - 1].len += fl[i].len;
+          for (j = i; j + 1 < nfree; j++) fl[j] = fl[j + 1];
+          nfree--;
+        }
+      }
+    /* a free block that ends at the top shrinks the arena instead */
+    if (nfree > 0 && fl[nfree - 1].off + fl[nfree - 1].len == top) {
+      top = fl[nfree - 1].off;
+      nfree--;
+    }
+    for (k = P->level_ptr[lev]; k < P->level_ptr[lev + 1]; k++) {
+      int sn = P->order[k], i, j;
+      int64_t u = (int64_t)(S->sn_up_size[sn] - S->sn_size[sn]), need = u * u;
+      if (need == 0) { P->off_U[sn] = 0; continue; }
+      for (i = 0; i < nfree && fl[i].len < need; i++) {}
+      if (i < nfree) {
+        P->off_U[sn] = fl[i].off;
+        fl[i].off += need;
+        fl[i].len -= need;
+        if (fl[i].len == 0) {
+          for (j = i; j + 1 < nfree; j++) fl[j] = fl[j + 1];
+          nfree--;
+        }
+      } else {
+        P->off_U[sn] = top;
+        top += need;
+        if (top > peak) peak = top;
+      }
+    }
+  }
+  tamd_free(fl);
+  tamd_free(dies);
+  tamd_free(dptr);
+  *total = peak;
+  return 0;
+}
+
 int tamd_plan_build(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_plan* P)
 {
   int nsn = S->n_sn, k, sn, c, nlive = 0, lev;
@@ -529,12 +627,11 @@ int tamd_plan_build(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_plan
     u = m - s;
     P->off_L[sn] = offl;
     offl += (int64_t)m * s;
-    P->off_U[sn] = offu;
-    offu += (int64_t)u * u;
     if (m > P->max_front) P->max_front = m;
     if (P->sn_parent[sn] < nsn) nrel += u;
   }
   P->L_elems = offl;
+  if (plan_update_blocks(S, P, &offu) != 0) goto fail;
   P->U_elems = offu;
   if (nrel > 0x7fffffff) goto fail;
   P->rel = (int*)tamd_malloc(sizeof(int) * (size_t)(nrel + 1));

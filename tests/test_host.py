@@ -288,3 +288,39 @@ def test_numeric_fails_loudly_without_gpu():
     perm, inv = T.order(A, "genmmd")
     PA = T.permute(A, perm, inv)
     assert not T.lib.taucs_ccs_factor_llt_mf(PA.ptr)
+
+
+@pytest.mark.parametrize("kind,side", [("2d", 60), ("3d", 20), ("3d", 34)])
+def test_update_blocks_with_overlapping_lifetimes_do_not_share_memory(kind, side):
+    """An update block is written at its front's level and read at its
+    parent's level; blocks alive at a common level must be disjoint, every
+    block lies inside the arena, and panels tile the L arena exactly."""
+    A = T.mesh2d(side) if kind == "2d" else T.mesh3d(side)
+    perm, inv = T.order(A, "genmmd")
+    F = T.factor_symbolic(T.permute(A, perm, inv))
+    fi = F.info()
+    S = F.supernodes()
+    sch = F.schedule()
+    ns = fi.n_sn
+    live = np.where(sch["level"] >= 0)[0]
+    assert len(live) == fi.n_live
+    s = S["sn_size"][:ns]
+    m = S["sn_up_size"][:ns]
+    u = (m - s).astype(np.int64)
+    lev, par, oU, oL = sch["level"], sch["parent"], sch["off_U"], sch["off_L"]
+    top = np.where(par < ns, lev[np.minimum(par, ns - 1)], lev)
+    blocks = [(int(lev[k]), int(top[k]), int(oU[k]), int(oU[k] + u[k] * u[k]))
+              for k in live if u[k] > 0]
+    U_elems = fi.U_bytes // 8
+    assert all(0 <= b[2] and b[3] <= U_elems for b in blocks)
+    assert max(b[3] for b in blocks) == U_elems
+    for l in range(fi.n_levels):
+        alive = sorted((b[2], b[3]) for b in blocks if b[0] <= l <= b[1])
+        for (a0, a1), (b0, b1) in zip(alive, alive[1:]):
+            assert a1 <= b0, (l, a0, a1, b0, b1)
+    # panels: disjoint and contiguous in level order
+    pl = sorted((int(oL[k]), int(oL[k] + m[k] * s[k])) for k in live)
+    assert pl[0][0] == 0 and pl[-1][1] == fi.L_bytes // 8
+    assert all(a[1] == b[0] for a, b in zip(pl, pl[1:]))
+    # reuse pays: far below the sum of all update blocks
+    assert U_elems < sum(int(x) * int(x) for x in u[live])
