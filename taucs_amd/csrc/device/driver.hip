@@ -16,6 +16,7 @@
 
 namespace tamd {
 template <typename T> __global__ void k_fill(T*, long long);
+template <typename T> __global__ void k_fill_ranges(const ZeroTask*, T*);
 template <typename T> __global__ void k_assemble(int, const int*, const long long*, const double*, T*);
 template <typename T> __global__ void k_front_small(const FrontDesc*, const int*, T*, T*, T*, int*);
 template <typename T> __global__ void k_potrf_diag(const FrontDesc*, const PanelTask*, T*, T*, int*);
@@ -71,6 +72,7 @@ struct tamd_dev {
   GemmTask* d_trsm = nullptr;   // panel solves as GEMMs with the block inverses
   GemmTask* d_gemm = nullptr;
   EATask* d_ea = nullptr;
+  ZeroTask* d_zero = nullptr;
   PanelTask* d_sv_tasks = nullptr;  // solve tasks
   std::vector<Op> sv_ops;
   int* d_info = nullptr;
@@ -224,7 +226,7 @@ extern "C" void tamd_dev_free(tamd_factor* F)
   tamd_dev_free_numeric(F);
   free_ptr(D->d_fronts); free_ptr(D->d_rows); free_ptr(D->d_rel); free_ptr(D->d_colptr);
   free_ptr(D->d_adest); free_ptr(D->d_small); free_ptr(D->d_potrf); free_ptr(D->d_trsm);
-  free_ptr(D->d_gemm); free_ptr(D->d_ea); free_ptr(D->d_sv_tasks);
+  free_ptr(D->d_gemm); free_ptr(D->d_ea); free_ptr(D->d_sv_tasks); free_ptr(D->d_zero);
   free_ptr(D->d_info); free_ptr(D->d_avals); free_ptr(D->d_x); free_ptr(D->d_b);
   if (D->stream) (void)hipStreamDestroy(D->stream);
   delete D;
@@ -339,6 +341,25 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
   std::vector<GemmTask> trsm;
   std::vector<GemmTask> gemm;
   std::vector<EATask> ea;
+  std::vector<ZeroTask> zero;
+  // update blocks by the level that zeroes them (global U offsets)
+  std::vector<std::vector<ZeroTask>> zero_at(nlev);
+  {
+    long long ub = 0;
+    for (size_t q = 0; q < sys.size(); q++) {
+      const tamd_symbolic* S = sys[q].S;
+      const tamd_plan* P = sys[q].P;
+      for (int k = 0; k < P->n_live; k++) {
+        int sn = P->order[k];
+        long long u = S->sn_up_size[sn] - S->sn_size[sn];
+        if (u <= 0) continue;
+        long long off = ub + P->off_U[sn], len = u * u;
+        for (long long c = 0; c < len; c += 65536)
+          zero_at[P->u_start[sn]].push_back({off + c, std::min(65536LL, len - c)});
+      }
+      ub += P->U_elems;
+    }
+  }
   D->lvl_ptr = lvl;
   D->ops.clear();
   D->ops.push_back({TAMD_OP_ASSEMBLE, 0, 0, -1});
@@ -346,6 +367,10 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
   D->assembly_bytes = 12.0 * (double)nnz_tot;
   for (int l = 0; l < nlev; l++) {
     int b0 = lvl[l], b1 = lvl[l + 1];
+    if (!zero_at[l].empty()) {
+      D->ops.push_back({TAMD_OP_ZERO, (int)zero.size(), (int)zero_at[l].size(), l});
+      zero.insert(zero.end(), zero_at[l].begin(), zero_at[l].end());
+    }
     int st = (int)small.size();
     int maxsteps = 0;
     for (int k = b0; k < b1; k++) {
@@ -560,6 +585,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
                     + sizeof(int) * (small.size() + colptr.size() + 1)
                     + sizeof(PanelTask) * (potrf.size() + svt.size())
                     + sizeof(GemmTask) * (trsm.size() + gemm.size()) + sizeof(EATask) * ea.size()
+                    + sizeof(ZeroTask) * zero.size()
                     + sizeof(double) * (size_t)nnz_tot;
   if (dry) return 0;
   if (upload(&D->d_fronts, fr, D->stream) || upload(&D->d_rows, rows, D->stream)
@@ -567,6 +593,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       || upload(&D->d_small, small, D->stream) || upload(&D->d_potrf, potrf, D->stream)
       || upload(&D->d_trsm, trsm, D->stream) || upload(&D->d_gemm, gemm, D->stream)
       || upload(&D->d_ea, ea, D->stream) || upload(&D->d_sv_tasks, svt, D->stream)
+      || upload(&D->d_zero, zero, D->stream)
       || upload(&D->d_colptr, colptr, D->stream)
       || dev_malloc((void**)&D->d_info, sizeof(int))
       || dev_malloc((void**)&D->d_avals, sizeof(double) * (nnz_tot > 0 ? nnz_tot : 1)))
@@ -676,13 +703,14 @@ static int launch_numeric(tamd_dev* D, bool bf16upd, bool prof)
     HIPCHECK(hipEventCreate(&a1));
   }
   hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(D->L_elems, 256)), dim3(256), 0, st, L, D->L_elems);
-  if (D->U_elems > 0)
-    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(D->U_elems, 256)), dim3(256), 0, st, U, D->U_elems);
   hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, st, D->d_info, 1LL);
-  int launches = 3;
+  int launches = 2;
   for (const Op& op : D->ops) {
     if (prof) HIPCHECK(hipEventRecord(a0, st));
     switch (op.kind) {
+      case TAMD_OP_ZERO:
+        hipLaunchKernelGGL(k_fill_ranges<T>, dim3(op.count), dim3(256), 0, st, D->d_zero + op.start, U);
+        break;
       case TAMD_OP_ASSEMBLE:
         hipLaunchKernelGGL(k_assemble<T>, dim3(grid_for(D->ncol, 128)), dim3(128), 0, st, D->ncol,
                            D->d_colptr, D->d_adest, D->d_avals, L);
@@ -770,7 +798,7 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
       D->num_graph_key = key;
     }
     HIPCHECK(hipGraphLaunch(D->num_graph, st));
-    D->timing.launches = (int)D->ops.size() + 3;
+    D->timing.launches = (int)D->ops.size() + 2;
   }
   HIPCHECK(hipGetLastError());
   if (prof) {

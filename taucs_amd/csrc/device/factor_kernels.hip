@@ -16,6 +16,15 @@ __global__ void k_fill(T* __restrict__ p, long long n)
   for (; i < n; i += stride) p[i] = (T)0;
 }
 
+// Zero the update blocks whose life starts at this level (one task = one
+// block range of at most 64K elements).
+template <typename T>
+__global__ void k_fill_ranges(const ZeroTask* __restrict__ tasks, T* __restrict__ U)
+{
+  const ZeroTask t = tasks[blockIdx.x];
+  for (long long i = threadIdx.x; i < t.len; i += blockDim.x) U[t.off + i] = (T)0;
+}
+
 // One thread per A column, entries in stored order: duplicates (if any) are
 // summed deterministically.
 template <typename T>
@@ -374,6 +383,7 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
 
 #define TAMD_INST(T)                                                                   \
   template __global__ void k_fill<T>(T*, long long);                                   \
+  template __global__ void k_fill_ranges<T>(const ZeroTask*, T*);                       \
   template __global__ void k_assemble<T>(int, const int*, const long long*,             \
                                          const double*, T*);                            \
   template __global__ void k_front_small<T>(const FrontDesc*, const int*, T*, T*, T*, int*); \

@@ -4,7 +4,9 @@
 //   L  panels: supernode f owns m_f x s_f at L + offL, leading dim m_f = s_f+u_f;
 //              rows 0..s-1 are the diagonal block (F1), rows s..m-1 the
 //              sub-diagonal block (F2) of src/taucs_sn_llt.c:1086-1194.
-//   U  update (contribution) blocks: u_f x u_f at U + offU, lower part used.
+//   U  update (contribution) blocks: u_f x u_f at U + offU, lower part used;
+//      blocks of disjoint lifetimes share space, each is zeroed (ZeroTask)
+//      at the start of the level where its first writer runs.
 //   W  solve workspace: m_f x nrhs per front.
 // Work is described by task arrays built once on the host per level schedule;
 // every kernel reads its task by blockIdx and has no inter-block dependence.
@@ -45,6 +47,10 @@ struct PanelTask {  // one diagonal block, or one row block of the panel
   int r0, nr;       // rows (or columns) r0 .. r0+nr-1
 };
 
+struct ZeroTask {   // U[off .. off+len) <- 0
+  long long off, len;
+};
+
 struct EATask {     // extend-add columns j0..j1-1 of a child's update block
   int child;
   int j0, j1;
@@ -57,3 +63,4 @@ struct EATask {     // extend-add columns j0..j1-1 of a child's update block
 #define TAMD_OP_TRSM     4
 #define TAMD_OP_GEMM     5
 #define TAMD_OP_EXTEND   6
+#define TAMD_OP_ZERO     7

@@ -58,6 +58,9 @@ typedef struct {
   int*     level_ptr;     /* [n_levels+1] */
   int64_t* off_L;         /* [n_sn] element offset of the (s+u) x s panel */
   int64_t* off_U;         /* [n_sn] element offset of the u x u update block */
+  int*     u_start;       /* [n_sn] level whose start zeroes the update block: the
+                             lowest child level (children extend-add into it), else
+                             the front's own level; it is dead after its own level */
   int64_t  L_elems;
   int64_t  U_elems;
   int*     rel_ptr;       /* [n_sn+1] offsets into rel */

@@ -416,6 +416,7 @@ void tamd_plan_free(tamd_plan* P)
   tamd_free(P->level_ptr);
   tamd_free(P->off_L);
   tamd_free(P->off_U);
+  tamd_free(P->u_start);
   tamd_free(P->rel_ptr);
   tamd_free(P->rel);
   tamd_free(P->sib_rank);
@@ -458,78 +459,87 @@ int tamd_plan_amap(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_plan*
 }
 
 /* Offsets of the update blocks U_sn (u x u, u = m - s) in one arena whose
-   blocks are reused once dead.  U_sn is written while level(sn) is factored
-   and read by the extend-add into its parent during level(parent); it is dead
-   from level(parent) + 1 on, so its space is handed to fronts of later
-   levels (first fit over a coalescing free list, deterministic).  The
-   reference keeps each update matrix on the heap from its front's return
-   until the parent has assembled it (src/taucs_sn_llt.c:1925-1940); a level
-   schedule gives the same lifetimes per level. */
+   space is reused once a block is dead.  Children extend-add into U_sn at the
+   end of their own levels, the front's SYRK updates it during level(sn), and
+   its own extend-add into the parent reads it at the end of level(sn); so U_sn
+   lives from u_start(sn) (the lowest child level, or level(sn) without
+   children) through level(sn), and is zeroed when its life starts.  Blocks
+   dead after level l-1 are handed to blocks starting at level l (first fit
+   over a coalescing free list; deterministic).  The reference keeps each
+   update matrix on the heap from its front's return until the parent has
+   assembled it (src/taucs_sn_llt.c:1925-1940); here the parent's block exists
+   already while its children assemble into it. */
 typedef struct { int64_t off, len; } uspan;
+
+static int uspan_free(uspan** fl, int* nfree, int* cap, uspan b)
+{
+  int i, j;
+  if (*nfree == *cap) {
+    uspan* nf = (uspan*)tamd_realloc(*fl, sizeof(uspan) * (size_t)(*cap) * 2);
+    if (!nf) return -1;
+    *fl = nf;
+    *cap *= 2;
+  }
+  for (i = *nfree; i > 0 && (*fl)[i - 1].off > b.off; i--) (*fl)[i] = (*fl)[i - 1];
+  (*fl)[i] = b;
+  (*nfree)++;
+  if (i + 1 < *nfree && (*fl)[i].off + (*fl)[i].len == (*fl)[i + 1].off) {
+    (*fl)[i].len += (*fl)[i + 1].len;
+    for (j = i + 1; j + 1 < *nfree; j++) (*fl)[j] = (*fl)[j + 1];
+    (*nfree)--;
+  }
+  if (i > 0 && (*fl)[i - 1].off + (*fl)[i - 1].len == (*fl)[i].off) {
+    (*fl)[i - 1].len += (*fl)[i].len;
+    for (j = i; j + 1 < *nfree; j++) (*fl)[j] = (*fl)[j + 1];
+    (*nfree)--;
+  }
+  return 0;
+}
 
 static int plan_update_blocks(const tamd_symbolic* S, tamd_plan* P, int64_t* total)
 {
-  int nsn = S->n_sn, nl = P->n_levels, k, lev;
+  int nsn = S->n_sn, nl = P->n_levels, k, lev, e;
   int64_t top = 0, peak = 0;   /* current end of the used space, and its maximum */
   int nfree = 0, cap = 64;
   uspan* fl = (uspan*)tamd_malloc(sizeof(uspan) * (size_t)cap);
-  int* dies = (int*)tamd_malloc(sizeof(int) * (size_t)(P->n_live + 1));   /* fronts by death level */
-  int* dptr = (int*)tamd_calloc((size_t)nl + 2, sizeof(int));
-  if (!fl || !dies || !dptr) { tamd_free(fl); tamd_free(dies); tamd_free(dptr); return -1; }
+  int* byst = (int*)tamd_malloc(sizeof(int) * (size_t)(P->n_live + 1));  /* by start level */
+  int* sptr = (int*)tamd_calloc((size_t)nl + 2, sizeof(int));
+  int* fill = (int*)tamd_malloc(sizeof(int) * (size_t)(nl + 1));
+  P->u_start = (int*)tamd_malloc(sizeof(int) * (size_t)(nsn + 1));
+  if (!fl || !byst || !sptr || !fill || !P->u_start) goto fail;
+  for (k = 0; k <= nsn; k++) P->u_start[k] = -1;
   for (k = 0; k < P->n_live; k++) {
-    int sn = P->order[k], p = P->sn_parent[sn];
-    if (p < nsn) dptr[P->sn_level[p] + 1]++;
+    int sn = P->order[k], c, st = P->sn_level[sn];
+    for (c = S->first_child[sn]; c != -1; c = S->next_child[c])
+      if (P->sn_level[c] < st) st = P->sn_level[c];
+    P->u_start[sn] = st;
+    sptr[st + 1]++;
   }
-  for (lev = 0; lev <= nl; lev++) dptr[lev + 1] += dptr[lev];
-  {
-    int* fill = (int*)tamd_malloc(sizeof(int) * (size_t)(nl + 1));
-    if (!fill) { tamd_free(fl); tamd_free(dies); tamd_free(dptr); return -1; }
-    memcpy(fill, dptr, sizeof(int) * (size_t)(nl + 1));
-    for (k = 0; k < P->n_live; k++) {
-      int sn = P->order[k], p = P->sn_parent[sn];
-      if (p < nsn) dies[fill[P->sn_level[p]]++] = sn;
-    }
-    tamd_free(fill);
+  for (lev = 0; lev <= nl; lev++) sptr[lev + 1] += sptr[lev];
+  memcpy(fill, sptr, sizeof(int) * (size_t)(nl + 1));
+  for (k = 0; k < P->n_live; k++) {   /* level order inside each start level */
+    int sn = P->order[k];
+    byst[fill[P->u_start[sn]]++] = sn;
   }
   for (lev = 0; lev < nl; lev++) {
-    int e;
-    /* blocks consumed by the extend-adds of level lev-1 are free now */
+    /* blocks whose own level was lev-1 are dead now */
     if (lev > 0)
-      for (e = dptr[lev - 1]; e < dptr[lev]; e++) {
-        int sn = dies[e], i, j;
+      for (k = P->level_ptr[lev - 1]; k < P->level_ptr[lev]; k++) {
+        int sn = P->order[k];
         int64_t u = (int64_t)(S->sn_up_size[sn] - S->sn_size[sn]);
         uspan b;
         if (u == 0) continue;
         b.off = P->off_U[sn];
         b.len = u * u;
-        /* insert sorted by offset, then merge with neighbours */
-        if (nfree == cap) {
-          uspan* nf = (uspan*)tamd_realloc(fl, sizeof(uspan) * (size_t)cap * 2);
-          if (!nf) { tamd_free(fl); tamd_free(dies); tamd_free(dptr); return -1; }
-          fl = nf;
-          cap *= 2;
-        }
-        for (i = nfree; i > 0 && fl[i - 1].off > b.off; i--) fl[i] = fl[i - 1];
-        fl[i] = b;
-        nfree++;
-        if (i + 1 < nfree && fl[i].off + fl[i].len == fl[i + 1].off) {
-          fl[i].len += fl[i + 1].len;
-          for (j = i + 1; j + 1 < nfree; j++) fl[j] = fl[j + 1];
-          nfree--;
-        }
-        if (i > 0 && fl[i - 1].off + fl[i - 1].len == fl[i].off) {
-          fl[i - 1].len += fl[i].len;
-          for (j = i; j + 1 < nfree; j++) fl[j] = fl[j + 1];
-          nfree--;
-        }
+        if (uspan_free(&fl, &nfree, &cap, b) != 0) goto fail;
       }
-    /* a free block that ends at the top shrinks the arena instead */
+    /* a free block that ends at the top shrinks the used space instead */
     if (nfree > 0 && fl[nfree - 1].off + fl[nfree - 1].len == top) {
       top = fl[nfree - 1].off;
       nfree--;
     }
-    for (k = P->level_ptr[lev]; k < P->level_ptr[lev + 1]; k++) {
-      int sn = P->order[k], i, j;
+    for (e = sptr[lev]; e < sptr[lev + 1]; e++) {
+      int sn = byst[e], i, j;
       int64_t u = (int64_t)(S->sn_up_size[sn] - S->sn_size[sn]), need = u * u;
       if (need == 0) { P->off_U[sn] = 0; continue; }
       for (i = 0; i < nfree && fl[i].len < need; i++) {}
@@ -549,10 +559,17 @@ static int plan_update_blocks(const tamd_symbolic* S, tamd_plan* P, int64_t* tot
     }
   }
   tamd_free(fl);
-  tamd_free(dies);
-  tamd_free(dptr);
+  tamd_free(byst);
+  tamd_free(sptr);
+  tamd_free(fill);
   *total = peak;
   return 0;
+fail:
+  tamd_free(fl);
+  tamd_free(byst);
+  tamd_free(sptr);
+  tamd_free(fill);
+  return -1;
 }
 
 int tamd_plan_build(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_plan* P)

@@ -292,9 +292,11 @@ def test_numeric_fails_loudly_without_gpu():
 
 @pytest.mark.parametrize("kind,side", [("2d", 60), ("3d", 20), ("3d", 34)])
 def test_update_blocks_with_overlapping_lifetimes_do_not_share_memory(kind, side):
-    """An update block is written at its front's level and read at its
-    parent's level; blocks alive at a common level must be disjoint, every
-    block lies inside the arena, and panels tile the L arena exactly."""
+    """Children extend-add into a front's update block at the end of their own
+    levels and the front's own extend-add reads it at the end of its level, so
+    a block lives from its lowest child's level through its own.  Blocks alive
+    at a common level must be disjoint, every block lies inside the arena, and
+    panels tile the L arena exactly."""
     A = T.mesh2d(side) if kind == "2d" else T.mesh3d(side)
     perm, inv = T.order(A, "genmmd")
     F = T.factor_symbolic(T.permute(A, perm, inv))
@@ -308,8 +310,11 @@ def test_update_blocks_with_overlapping_lifetimes_do_not_share_memory(kind, side
     m = S["sn_up_size"][:ns]
     u = (m - s).astype(np.int64)
     lev, par, oU, oL = sch["level"], sch["parent"], sch["off_U"], sch["off_L"]
-    top = np.where(par < ns, lev[np.minimum(par, ns - 1)], lev)
-    blocks = [(int(lev[k]), int(top[k]), int(oU[k]), int(oU[k] + u[k] * u[k]))
+    start = lev.copy()
+    for k in live:
+        if par[k] < ns:
+            start[par[k]] = min(start[par[k]], lev[k])
+    blocks = [(int(start[k]), int(lev[k]), int(oU[k]), int(oU[k] + u[k] * u[k]))
               for k in live if u[k] > 0]
     U_elems = fi.U_bytes // 8
     assert all(0 <= b[2] and b[3] <= U_elems for b in blocks)
