@@ -113,6 +113,12 @@ void  taucs_amd_batch_free(void* B);
 
 /* hipGraph capture of the numeric schedule and of the solves (default on) */
 void  taucs_amd_set_graphs(int on);
+/* Panel lookahead: the trailing update of a panel step runs on a second stream
+   while the next panel is factored (default on; results are identical). */
+void  taucs_amd_set_lookahead(int on);
+/* Diagonal-block POTRF kernel: 1 = one wave, register broadcasts (default);
+   0 = 256 threads through LDS. */
+void  taucs_amd_set_potrf_kernel(int wave);
 
 /* ---- device buffers (for right-hand sides kept in HBM) ------------------- */
 void* taucs_amd_device_malloc(size_t bytes);

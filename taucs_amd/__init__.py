@@ -96,6 +96,8 @@ _sigs = {
                                           P(RefineInfo)]),
     "taucs_amd_set_profiling": (None, [C.c_int]),
     "taucs_amd_set_graphs": (None, [C.c_int]),
+    "taucs_amd_set_lookahead": (None, [C.c_int]),
+    "taucs_amd_set_potrf_kernel": (None, [C.c_int]),
     "taucs_amd_batch_factor_llt_mf": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int]),
     "taucs_amd_batch_numeric": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "taucs_amd_batch_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),

@@ -20,6 +20,7 @@ template <typename T> __global__ void k_fill_ranges(const ZeroTask*, T*);
 template <typename T> __global__ void k_assemble(int, const int*, const long long*, const double*, T*);
 template <typename T> __global__ void k_front_small(const FrontDesc*, const int*, T*, T*, T*, int*);
 template <typename T> __global__ void k_potrf_diag(const FrontDesc*, const PanelTask*, T*, T*, int*);
+template <typename T> __global__ void k_potrf_wave(const FrontDesc*, const PanelTask*, T*, T*, int*);
 template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*, const T*);
 __global__ void k_gemm_tile_bf16(const GemmTask*, const float*, float*, float*);
 template <typename T> __global__ void k_extend_add(const FrontDesc*, const EATask*, const int*, T*, T*);
@@ -53,6 +54,7 @@ struct Op {
   int kind;
   int start, count;  // range in the task array of that kind
   int level;
+  int lane = 0;      // 1: may run on the side stream (the lookahead's trailing update)
 };
 
 // solve op kinds
@@ -60,6 +62,8 @@ enum { SV_INIT, SV_FSTEP, SV_FRECT, SV_FRED, SV_EXT, SV_GATHER, SV_BCOLS, SV_BRE
 
 struct tamd_dev {
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;    // second stream of the panel lookahead
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int elem = 8;  // bytes per factor element
   // structure (uploaded once)
   FrontDesc* d_fronts = nullptr;
@@ -117,6 +121,12 @@ struct tamd_dev {
 
 static int g_graphs = 1;
 extern "C" void taucs_amd_set_graphs(int on) { g_graphs = on; }
+static int g_lookahead = 1;
+extern "C" void taucs_amd_set_lookahead(int on) { g_lookahead = on; }
+// Diagonal-block kernel: 1 = one wave with register broadcasts, 0 = 256
+// threads through LDS (kept for comparison).
+static int g_potrf_wave = 1;
+extern "C" void taucs_amd_set_potrf_kernel(int wave) { g_potrf_wave = wave; }
 
 static void drop_solve_graphs(tamd_dev* D)
 {
@@ -228,6 +238,9 @@ extern "C" void tamd_dev_free(tamd_factor* F)
   free_ptr(D->d_adest); free_ptr(D->d_small); free_ptr(D->d_potrf); free_ptr(D->d_trsm);
   free_ptr(D->d_gemm); free_ptr(D->d_ea); free_ptr(D->d_sv_tasks); free_ptr(D->d_zero);
   free_ptr(D->d_info); free_ptr(D->d_avals); free_ptr(D->d_x); free_ptr(D->d_b);
+  if (D->ev_fork) (void)hipEventDestroy(D->ev_fork);
+  if (D->ev_join) (void)hipEventDestroy(D->ev_join);
+  if (D->side) (void)hipStreamDestroy(D->side);
   if (D->stream) (void)hipStreamDestroy(D->stream);
   delete D;
   F->dev = nullptr;
@@ -378,8 +391,13 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       else maxsteps = std::max(maxsteps, (fr[k].s + NB - 1) / NB);
     }
     if ((int)small.size() > st) D->ops.push_back({TAMD_OP_SMALL, st, (int)small.size() - st, l});
+    // Lookahead: the trailing update of a step is split into the tiles of the
+    // next panel (main stream) and the rest (side stream), so the next
+    // panel's POTRF and TRSM overlap the rest of this step's update.
+    std::vector<GemmTask> rest;
     for (int step = 0; step < maxsteps; step++) {
       int p0 = (int)potrf.size(), t0 = (int)trsm.size(), g0 = (int)gemm.size();
+      rest.clear();
       for (int k = b0; k < b1; k++) {
         const FrontDesc& d = fr[k];
         if (d.m <= SMALL_M) continue;
@@ -412,13 +430,19 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
             g.N = std::min(64, d.s - c0);
             g.K = kb;
             g.flags = (r0 == c0) ? 2 : 0;
-            gemm.push_back(g);
+            (c0 == k0 + kb ? gemm : rest).push_back(g);
             D->gemm_flops += tile_flops(g);
           }
       }
       if ((int)potrf.size() > p0) D->ops.push_back({TAMD_OP_POTRF, p0, (int)potrf.size() - p0, l});
       if ((int)trsm.size() > t0) D->ops.push_back({TAMD_OP_TRSM, t0, (int)trsm.size() - t0, l});
       if ((int)gemm.size() > g0) D->ops.push_back({TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l});
+      if (!rest.empty()) {
+        Op o = {TAMD_OP_GEMM, (int)gemm.size(), (int)rest.size(), l};
+        o.lane = 1;
+        D->ops.push_back(o);
+        gemm.insert(gemm.end(), rest.begin(), rest.end());
+      }
     }
     // SYRK of the update blocks, K = s
     {
@@ -602,6 +626,18 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
   return 0;
 }
 
+// The factor's own stream plus the side stream of the lookahead, both
+// non-blocking and at the default priority.
+static int create_streams(tamd_dev* D)
+{
+  if (hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking) != hipSuccess
+      || hipStreamCreateWithFlags(&D->side, hipStreamNonBlocking) != hipSuccess
+      || hipEventCreateWithFlags(&D->ev_fork, hipEventDisableTiming) != hipSuccess
+      || hipEventCreateWithFlags(&D->ev_join, hipEventDisableTiming) != hipSuccess)
+    return -1;
+  return 0;
+}
+
 // Device schedule of `count` independent systems sharing the launches of
 // every level; the systems' vectors are concatenated at offsets xoff[q].
 extern "C" int tamd_dev_batch_setup(tamd_factor* FB, int count, tamd_factor* const* F,
@@ -613,7 +649,7 @@ extern "C" int tamd_dev_batch_setup(tamd_factor* FB, int count, tamd_factor* con
   }
   tamd_dev* D = new tamd_dev();
   FB->dev = D;
-  if (hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (create_streams(D) != 0) {
     tamd_dev_free(FB);
     return -1;
   }
@@ -638,7 +674,7 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
   }
   tamd_dev* D = new tamd_dev();
   F->dev = D;
-  if (hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (create_streams(D) != 0) {
     tamd_dev_free(F);
     return -1;
   }
@@ -705,7 +741,22 @@ static int launch_numeric(tamd_dev* D, bool bf16upd, bool prof)
   hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(D->L_elems, 256)), dim3(256), 0, st, L, D->L_elems);
   hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, st, D->d_info, 1LL);
   int launches = 2;
+  // Side-stream ops fork from the main stream after everything issued so far;
+  // the main stream joins them before any op other than the next panel's
+  // POTRF/TRSM (which touch other columns than the side stream's tiles).
+  const bool la = !prof && g_lookahead && D->side;
+  bool pending = false;
+  hipStream_t main_st = st;
   for (const Op& op : D->ops) {
+    st = main_st;
+    if (la && op.lane == 1) {
+      HIPCHECK(hipEventRecord(D->ev_fork, main_st));
+      HIPCHECK(hipStreamWaitEvent(D->side, D->ev_fork, 0));
+      st = D->side;
+    } else if (pending && op.kind != TAMD_OP_POTRF && op.kind != TAMD_OP_TRSM) {
+      HIPCHECK(hipStreamWaitEvent(main_st, D->ev_join, 0));
+      pending = false;
+    }
     if (prof) HIPCHECK(hipEventRecord(a0, st));
     switch (op.kind) {
       case TAMD_OP_ZERO:
@@ -720,8 +771,12 @@ static int launch_numeric(tamd_dev* D, bool bf16upd, bool prof)
                            D->d_small + op.start, L, U, Iv, D->d_info);
         break;
       case TAMD_OP_POTRF:
-        hipLaunchKernelGGL(k_potrf_diag<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts,
-                           D->d_potrf + op.start, L, Iv, D->d_info);
+        if (g_potrf_wave)
+          hipLaunchKernelGGL(k_potrf_wave<T>, dim3(op.count), dim3(64), 0, st, D->d_fronts,
+                             D->d_potrf + op.start, L, Iv, D->d_info);
+        else
+          hipLaunchKernelGGL(k_potrf_diag<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts,
+                             D->d_potrf + op.start, L, Iv, D->d_info);
         break;
       case TAMD_OP_TRSM:
         hipLaunchKernelGGL(k_gemm_tile<T>, dim3(op.count), dim3(256), 0, st,
@@ -741,6 +796,10 @@ static int launch_numeric(tamd_dev* D, bool bf16upd, bool prof)
         break;
     }
     launches++;
+    if (la && op.lane == 1) {
+      HIPCHECK(hipEventRecord(D->ev_join, D->side));
+      pending = true;
+    }
     if (prof) {
       float ms = 0;
       HIPCHECK(hipEventRecord(a1, st));
@@ -754,6 +813,7 @@ static int launch_numeric(tamd_dev* D, bool bf16upd, bool prof)
       }
     }
   }
+  if (pending) HIPCHECK(hipStreamWaitEvent(main_st, D->ev_join, 0));
   if (prof) {
     (void)hipEventDestroy(a0);
     (void)hipEventDestroy(a1);

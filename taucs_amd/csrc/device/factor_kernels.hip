@@ -201,6 +201,84 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const FrontDesc* __restrict_
   if (tid == 0 && bad) atomicOr(info, 1);
 }
 
+// ---- one-wave Cholesky + inverse of a 64 x 64 diagonal block ----------------
+// Lane i holds row i of the block in 64 registers; every value another lane
+// needs is broadcast with v_readlane (the loops are fully unrolled, so every
+// register index and source lane is a constant), so the whole block is
+// factored and inverted without LDS and without a barrier.
+//   Cholesky (right-looking, src/taucs_sn_llt.c:1112-1120 calls dpotrf on
+//   the same block): for k: d = sqrt(a_kk); rows i > k scale a_ik by 1/d;
+//   a_ij -= a_ik a_jk for k < j <= i.
+//   Inverse in place, row sweep: after step t row t of X = L^{-1} is final:
+//   x_tc = x_tc / l_tt (c < t), x_tt = 1 / l_tt; rows i > t then take
+//   x_ic -= l_it x_tc (c <= t), reusing l_ic's register once l_ic is spent.
+// A kb < 64 block is padded with the identity, whose factor and inverse are
+// the identity, so the same fully unrolled code serves every block size.
+template <typename T>
+__device__ __forceinline__ T rlane(T v, int l);
+template <>
+__device__ __forceinline__ double rlane<double>(double v, int l)
+{
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+template <>
+__device__ __forceinline__ float rlane<float>(float v, int l)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__ fr,
+                                                   const PanelTask* __restrict__ tasks,
+                                                   T* __restrict__ L, T* __restrict__ Inv,
+                                                   int* __restrict__ info)
+{
+  const PanelTask t = tasks[blockIdx.x];
+  const FrontDesc d = fr[t.front];
+  const int m = d.m, kb = t.kb, lane = threadIdx.x;
+  T* gp = L + d.offL + t.k0 + (long long)t.k0 * m;
+  T r[64];
+#pragma unroll
+  for (int c = 0; c < 64; c++)
+    r[c] = (lane < kb && c < kb) ? gp[lane + (long long)c * m] : (lane == c ? (T)1 : (T)0);
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < 64; k++) {
+    T p = rlane(r[k], k);
+    if (!(p > (T)0)) { bad = true; p = (T)1; }
+    const T dk = sqrt(p), rdk = (T)1 / dk;
+    r[k] = lane == k ? dk : (lane > k ? r[k] * rdk : r[k]);
+#pragma unroll
+    for (int j = k + 1; j < 64; j++) {
+      const T ljk = rlane(r[k], j);
+      if (lane >= j) r[j] -= r[k] * ljk;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 64; c++)
+    if (lane < kb && c < kb && lane >= c) gp[lane + (long long)c * m] = r[c];
+#pragma unroll
+  for (int s = 0; s < 64; s++) {
+    // row s becomes final (all lanes stay active around the broadcasts)
+    const T rls = (T)1 / rlane(r[s], s);
+#pragma unroll
+    for (int c = 0; c < s; c++) r[c] = lane == s ? r[c] * rls : r[c];
+    const T lis = lane > s ? r[s] : (T)0;
+    r[s] = lane == s ? rls : (lane > s ? (T)0 : r[s]);
+#pragma unroll
+    for (int c = 0; c <= s; c++) {
+      const T xsc = rlane(r[c], s);
+      r[c] -= lis * xsc;
+    }
+  }
+  T* iv = Inv + d.offI + (long long)(t.k0 / 64) * 4096;
+#pragma unroll
+  for (int c = 0; c < 64; c++) iv[lane * 64 + c] = (lane < kb && c < kb && c <= lane) ? r[c] : (T)0;
+  if (lane == 0 && bad) atomicOr(info, 1);
+}
+
 // ---- MFMA tile GEMM ---------------------------------------------------------
 // 64x64 tile, 4 waves (2x2), each wave a 32x32 sub-tile of 2x2 16x16 MFMA
 // blocks.  K is staged through LDS in chunks of 16.  The MFMA is issued as
@@ -388,6 +466,7 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
                                          const double*, T*);                            \
   template __global__ void k_front_small<T>(const FrontDesc*, const int*, T*, T*, T*, int*); \
   template __global__ void k_potrf_diag<T>(const FrontDesc*, const PanelTask*, T*, T*, int*); \
+  template __global__ void k_potrf_wave<T>(const FrontDesc*, const PanelTask*, T*, T*, int*); \
   template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*);
 TAMD_INST(double)

@@ -94,3 +94,31 @@ def test_graph_replays_match_eager():
             G.free()
         finally:
             T.lib.taucs_amd_set_graphs(1)
+
+
+def test_lookahead_gives_identical_results():
+    """The panel lookahead only reorders independent launches across two
+    streams; factor and solution are bitwise those of the one-stream order."""
+    print("case: lookahead on vs off", flush=True)
+    A = T.mesh3d(24)
+    perm, inv = T.order(A, "genmmd")
+    PA = T.permute(A, perm, inv)
+    b = T.random_uniform(A.n, 77)
+    T.lib.taucs_amd_set_lookahead(0)
+    try:
+        F0 = T.factor_llt_mf(PA)
+        x0, d0 = F0.solve(b), F0.diag()
+        L0 = F0.to_ccs().arrays()[3]
+        F0.free()
+    finally:
+        T.lib.taucs_amd_set_lookahead(1)
+    for graphs in (1, 0):
+        T.lib.taucs_amd_set_graphs(graphs)
+        try:
+            F1 = T.factor_llt_mf(PA)
+            assert np.array_equal(F1.diag(), d0)
+            assert np.array_equal(F1.to_ccs().arrays()[3], L0)
+            assert np.array_equal(F1.solve(b), x0)
+            F1.free()
+        finally:
+            T.lib.taucs_amd_set_graphs(1)
