@@ -216,7 +216,7 @@ def bench_serving(T, requests=300, rate=40.0, slots=4, budget_gib=16.0, seed=3):
         srv = S.Server(hbm_budget=int(budget_gib * S.GIB), slots=slots, precision=prec, tol=1e-12)
         summ = S.run_mix(mix, srv)
         srv.close()
-        out[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in summ.items()}
+        out[name] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in summ.items()}
         log(f"serving {name}: {out[name]}")
     out["workload"] = (f"{requests} requests, Poisson {rate}/s: 70% 2D side U[32,256], 30% 3D side "
                        f"U[8,40], 1-16 rhs, 40% solve-only; {slots} slots, budget {budget_gib} GiB")

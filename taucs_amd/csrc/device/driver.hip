@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <tuple>
+#include <utility>
 #include <unordered_map>
 #include <vector>
 
@@ -94,6 +96,8 @@ struct tamd_dev {
   void* Inv = nullptr;           // inverses of the 64x64 diagonal blocks
   long long I_elems = 0;
   double* d_avals = nullptr;
+  const double* num_vals = nullptr;   // host values of the last numeric factorization
+  unsigned long long num_gen = 0;     // numeric factorizations so far
   long long L_elems = 0, U_elems = 0;
   // solve buffers
   void* W = nullptr;
@@ -108,6 +112,7 @@ struct tamd_dev {
   // captured launch sequences (replayed while the buffers they name live)
   hipGraphExec_t num_graph = nullptr;
   int num_graph_key = -1;
+  int num_graph_launches = 0;
   struct SolveGraph {
     hipGraphExec_t exec = nullptr;
     int nrhs = 0, prec = -1, ldx = 0, ldb = 0;
@@ -146,9 +151,9 @@ static void drop_graphs(tamd_dev* D)
 
 // Every device allocation of the library goes through dev_malloc/free_ptr,
 // which keep a ledger of live bytes (for admission by HBM budget) under one
-// mutex.  The same mutex is held from stream-capture begin to graph
-// instantiation, so no thread frees or allocates device memory (which may
-// synchronize the device) while another thread's stream is being captured.
+// mutex.  Graphs are built node by node (never by stream capture), so
+// threads working on different factors never constrain each other's HIP
+// calls.
 static std::mutex g_mem_mu;
 static std::unordered_map<void*, size_t> g_live;
 static size_t g_live_bytes = 0;
@@ -726,10 +731,95 @@ static inline unsigned grid_for(long long n, int block)
   return (unsigned)(g > 0 ? g : 1);
 }
 
+// Issues kernels either on streams (eager) or as kernel nodes of a hipGraph
+// built explicitly (no stream capture, so other threads' HIP calls never
+// interfere with it).  Lane 0 is the main chain; lane 1 ops fork from the
+// main chain's last op; `join` makes a main op wait for the side chain too.
+struct Emitter {
+  hipStream_t main_st = nullptr, side_st = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipGraph_t g = nullptr;
+  hipGraphNode_t last_main = nullptr, last_side = nullptr;
+  bool side_pending = false;
+  int count = 0;
+
+  template <typename... P, size_t... I>
+  static void ptrs_of(std::tuple<P...>& t, void** p, std::index_sequence<I...>)
+  {
+    ((p[I] = (void*)&std::get<I>(t)), ...);
+  }
+
+  template <typename... P, typename... A>
+  int kernel(int lane, bool join, void (*k)(P...), dim3 grid, dim3 block, A... a)
+  {
+    std::tuple<typename std::decay<P>::type...> vals(a...);
+    void* p[sizeof...(P) > 0 ? sizeof...(P) : 1];
+    ptrs_of(vals, p, std::index_sequence_for<P...>{});
+    count++;
+    if (g) {
+      hipGraphNode_t deps[2];
+      int nd = 0;
+      if (lane == 1) {
+        if (last_main) deps[nd++] = last_main;
+        if (last_side) deps[nd++] = last_side;
+      } else {
+        if (last_main) deps[nd++] = last_main;
+        if (join && last_side) deps[nd++] = last_side;
+      }
+      hipKernelNodeParams kp;
+      memset(&kp, 0, sizeof(kp));
+      kp.func = (void*)k;
+      kp.gridDim = grid;
+      kp.blockDim = block;
+      kp.sharedMemBytes = 0;
+      kp.kernelParams = p;
+      kp.extra = nullptr;
+      hipGraphNode_t node;
+      HIPCHECK(hipGraphAddKernelNode(&node, g, deps, nd, &kp));
+      if (lane == 1) {
+        last_side = node;
+      } else {
+        last_main = node;
+        if (join) last_side = nullptr;
+      }
+      return 0;
+    }
+    hipStream_t st = main_st;
+    if (lane == 1 && side_st) {
+      HIPCHECK(hipEventRecord(ev_fork, main_st));
+      HIPCHECK(hipStreamWaitEvent(side_st, ev_fork, 0));
+      st = side_st;
+    } else if (join && side_pending) {
+      HIPCHECK(hipStreamWaitEvent(main_st, ev_join, 0));
+      side_pending = false;
+    }
+    HIPCHECK(hipLaunchKernel((const void*)k, grid, block, p, 0, st));
+    if (lane == 1 && side_st) {
+      HIPCHECK(hipEventRecord(ev_join, side_st));
+      side_pending = true;
+    }
+    return 0;
+  }
+
+  // eager mode: the main stream waits for the side chain
+  int finish()
+  {
+    if (!g && side_pending) {
+      HIPCHECK(hipStreamWaitEvent(main_st, ev_join, 0));
+      side_pending = false;
+    }
+    return 0;
+  }
+};
+
+#define EMIT(...)                            \
+  do {                                       \
+    if (E.kernel(__VA_ARGS__) != 0) return -1; \
+  } while (0)
+
 template <typename T>
-static int launch_numeric(tamd_dev* D, bool bf16upd, bool prof)
+static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
 {
-  hipStream_t st = D->stream;
   T* L = (T*)D->L;
   T* U = (T*)D->U;
   T* Iv = (T*)D->Inv;
@@ -738,92 +828,90 @@ static int launch_numeric(tamd_dev* D, bool bf16upd, bool prof)
     HIPCHECK(hipEventCreate(&a0));
     HIPCHECK(hipEventCreate(&a1));
   }
-  hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(D->L_elems, 256)), dim3(256), 0, st, L, D->L_elems);
-  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, st, D->d_info, 1LL);
-  int launches = 2;
-  // Side-stream ops fork from the main stream after everything issued so far;
-  // the main stream joins them before any op other than the next panel's
-  // POTRF/TRSM (which touch other columns than the side stream's tiles).
+  EMIT(0, true, k_fill<T>, dim3(grid_for(D->L_elems, 256)), dim3(256), L, D->L_elems);
+  EMIT(0, true, k_fill<int>, dim3(1), dim3(64), D->d_info, 1LL);
+  // Side-lane ops (the lookahead's trailing update) may overlap the next
+  // panel's POTRF/TRSM, which touch other columns; every other op joins.
   const bool la = !prof && g_lookahead && D->side;
-  bool pending = false;
-  hipStream_t main_st = st;
   for (const Op& op : D->ops) {
-    st = main_st;
-    if (la && op.lane == 1) {
-      HIPCHECK(hipEventRecord(D->ev_fork, main_st));
-      HIPCHECK(hipStreamWaitEvent(D->side, D->ev_fork, 0));
-      st = D->side;
-    } else if (pending && op.kind != TAMD_OP_POTRF && op.kind != TAMD_OP_TRSM) {
-      HIPCHECK(hipStreamWaitEvent(main_st, D->ev_join, 0));
-      pending = false;
-    }
-    if (prof) HIPCHECK(hipEventRecord(a0, st));
+    const int lane = la ? op.lane : 0;
+    const bool join = !(op.kind == TAMD_OP_POTRF || op.kind == TAMD_OP_TRSM);
+    if (prof) HIPCHECK(hipEventRecord(a0, E.main_st));
     switch (op.kind) {
       case TAMD_OP_ZERO:
-        hipLaunchKernelGGL(k_fill_ranges<T>, dim3(op.count), dim3(256), 0, st, D->d_zero + op.start, U);
+        EMIT(lane, join, k_fill_ranges<T>, dim3(op.count), dim3(256),
+             (const ZeroTask*)(D->d_zero + op.start), U);
         break;
       case TAMD_OP_ASSEMBLE:
-        hipLaunchKernelGGL(k_assemble<T>, dim3(grid_for(D->ncol, 128)), dim3(128), 0, st, D->ncol,
-                           D->d_colptr, D->d_adest, D->d_avals, L);
+        EMIT(lane, join, k_assemble<T>, dim3(grid_for(D->ncol, 128)), dim3(128), D->ncol,
+             (const int*)D->d_colptr, (const long long*)D->d_adest, (const double*)D->d_avals, L);
         break;
       case TAMD_OP_SMALL:
-        hipLaunchKernelGGL(k_front_small<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts,
-                           D->d_small + op.start, L, U, Iv, D->d_info);
+        EMIT(lane, join, k_front_small<T>, dim3(op.count), dim3(256), (const FrontDesc*)D->d_fronts,
+             (const int*)(D->d_small + op.start), L, U, Iv, D->d_info);
         break;
       case TAMD_OP_POTRF:
         if (g_potrf_wave)
-          hipLaunchKernelGGL(k_potrf_wave<T>, dim3(op.count), dim3(64), 0, st, D->d_fronts,
-                             D->d_potrf + op.start, L, Iv, D->d_info);
+          EMIT(lane, join, k_potrf_wave<T>, dim3(op.count), dim3(64), (const FrontDesc*)D->d_fronts,
+               (const PanelTask*)(D->d_potrf + op.start), L, Iv, D->d_info);
         else
-          hipLaunchKernelGGL(k_potrf_diag<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts,
-                             D->d_potrf + op.start, L, Iv, D->d_info);
+          EMIT(lane, join, k_potrf_diag<T>, dim3(op.count), dim3(256), (const FrontDesc*)D->d_fronts,
+               (const PanelTask*)(D->d_potrf + op.start), L, Iv, D->d_info);
         break;
       case TAMD_OP_TRSM:
-        hipLaunchKernelGGL(k_gemm_tile<T>, dim3(op.count), dim3(256), 0, st,
-                           D->d_trsm + op.start, (const T*)L, L, U, (const T*)Iv);
+        EMIT(lane, join, k_gemm_tile<T>, dim3(op.count), dim3(256),
+             (const GemmTask*)(D->d_trsm + op.start), (const T*)L, L, U, (const T*)Iv);
         break;
       case TAMD_OP_GEMM:
         if (bf16upd)
-          hipLaunchKernelGGL(k_gemm_tile_bf16, dim3(op.count), dim3(256), 0, st,
-                             D->d_gemm + op.start, (const float*)L, (float*)L, (float*)U);
+          EMIT(lane, join, k_gemm_tile_bf16, dim3(op.count), dim3(256),
+               (const GemmTask*)(D->d_gemm + op.start), (const float*)L, (float*)L, (float*)U);
         else
-          hipLaunchKernelGGL(k_gemm_tile<T>, dim3(op.count), dim3(256), 0, st,
-                             D->d_gemm + op.start, (const T*)L, L, U, (const T*)Iv);
+          EMIT(lane, join, k_gemm_tile<T>, dim3(op.count), dim3(256),
+               (const GemmTask*)(D->d_gemm + op.start), (const T*)L, L, U, (const T*)Iv);
         break;
       case TAMD_OP_EXTEND:
-        hipLaunchKernelGGL(k_extend_add<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts,
-                           D->d_ea + op.start, D->d_rel, L, U);
+        EMIT(lane, join, k_extend_add<T>, dim3(op.count), dim3(256), (const FrontDesc*)D->d_fronts,
+             (const EATask*)(D->d_ea + op.start), (const int*)D->d_rel, L, U);
         break;
-    }
-    launches++;
-    if (la && op.lane == 1) {
-      HIPCHECK(hipEventRecord(D->ev_join, D->side));
-      pending = true;
     }
     if (prof) {
       float ms = 0;
-      HIPCHECK(hipEventRecord(a1, st));
+      HIPCHECK(hipEventRecord(a1, E.main_st));
       HIPCHECK(hipEventSynchronize(a1));
       HIPCHECK(hipEventElapsedTime(&ms, a0, a1));
-      if (op.kind == TAMD_OP_ASSEMBLE || op.kind == TAMD_OP_EXTEND) D->timing.assembly_ms += ms;
-      else D->timing.front_ms += ms;
+      if (op.kind == TAMD_OP_ASSEMBLE || op.kind == TAMD_OP_EXTEND || op.kind == TAMD_OP_ZERO)
+        D->timing.assembly_ms += ms;
+      else
+        D->timing.front_ms += ms;
       if (op.kind == TAMD_OP_GEMM) {
         D->timing.gemm_ms += ms;
         D->timing.gemm_launches++;
       }
     }
   }
-  if (pending) HIPCHECK(hipStreamWaitEvent(main_st, D->ev_join, 0));
+  if (E.finish() != 0) return -1;
   if (prof) {
     (void)hipEventDestroy(a0);
     (void)hipEventDestroy(a1);
   }
-  D->timing.launches = launches;
+  D->timing.launches = E.count;
   return 0;
 }
 
-// Runs the whole numeric schedule.  Without profiling the launch sequence is
-// captured once into a hipGraph (kernel nodes only) and replayed afterwards.
+static Emitter eager_emitter(tamd_dev* D)
+{
+  Emitter E;
+  E.main_st = D->stream;
+  E.side_st = D->side;
+  E.ev_fork = D->ev_fork;
+  E.ev_join = D->ev_join;
+  return E;
+}
+
+// Runs the whole numeric schedule.  Without profiling the schedule is built
+// once into a hipGraph (kernel nodes only; the lookahead's two chains become
+// graph branches) and replayed afterwards.
 template <typename T>
 static int run_numeric(tamd_dev* D, bool bf16upd)
 {
@@ -836,29 +924,32 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
     HIPCHECK(hipEventCreate(&e1));
     HIPCHECK(hipEventRecord(e0, st));
   }
-  const int key = (int)sizeof(T) * 2 + (bf16upd ? 1 : 0);
+  const int key = (int)sizeof(T) * 2 + (bf16upd ? 1 : 0) + (g_lookahead ? 16 : 0)
+                  + (g_potrf_wave ? 32 : 0);
   if (prof || !g_graphs) {
-    if (launch_numeric<T>(D, bf16upd, prof) != 0) return -1;
+    Emitter E = eager_emitter(D);
+    if (launch_numeric<T>(D, E, bf16upd, prof) != 0) return -1;
   } else {
     if (!D->num_graph || D->num_graph_key != key) {
       if (D->num_graph) (void)hipGraphExecDestroy(D->num_graph);
       D->num_graph = nullptr;
-      hipGraph_t g = nullptr;
-      std::lock_guard<std::mutex> cap_lk(g_mem_mu);
-      HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      int rc = launch_numeric<T>(D, bf16upd, false);
-      HIPCHECK(hipStreamEndCapture(st, &g));
-      if (rc != 0) {
-        (void)hipGraphDestroy(g);
+      Emitter E;
+      E.main_st = st;
+      HIPCHECK(hipGraphCreate(&E.g, 0));
+      int rc = launch_numeric<T>(D, E, bf16upd, false);
+      hipError_t ie = rc == 0 ? hipGraphInstantiate(&D->num_graph, E.g, nullptr, nullptr, 0)
+                              : hipErrorUnknown;
+      (void)hipGraphDestroy(E.g);
+      if (ie != hipSuccess) {
+        D->num_graph = nullptr;
+        taucs_printf((char*)"taucs_amd: building the numeric graph failed\n");
         return -1;
       }
-      hipError_t ie = hipGraphInstantiate(&D->num_graph, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      HIPCHECK(ie);
       D->num_graph_key = key;
+      D->num_graph_launches = E.count;
     }
     HIPCHECK(hipGraphLaunch(D->num_graph, st));
-    D->timing.launches = (int)D->ops.size() + 2;
+    D->timing.launches = D->num_graph_launches;
   }
   HIPCHECK(hipGetLastError());
   if (prof) {
@@ -906,11 +997,25 @@ extern "C" int tamd_dev_numeric_values(tamd_factor* F, const double* vals)
 {
   tamd_dev* D = F->dev;
   if (!D || ensure_numeric_buffers(F) != 0) return -1;
-  tamd_refine_forget(F);
   HIPCHECK(hipMemcpyAsync(D->d_avals, vals, sizeof(double) * (size_t)D->nnz, hipMemcpyHostToDevice,
                           D->stream));
+  D->num_vals = vals;
+  D->num_gen++;
   if (F->precision == TAMD_PREC_FP64) return run_numeric<double>(D, false);
   return run_numeric<float>(D, F->precision == TAMD_PREC_BF16UPD);
+}
+
+// The device copy of the values of the last numeric factorization, the host
+// array they came from, and a counter that changes with every factorization.
+extern "C" int tamd_dev_values_of(tamd_factor* F, const double** d_avals, const double** host_src,
+                                  unsigned long long* gen)
+{
+  tamd_dev* D = F ? F->dev : nullptr;
+  if (!D || !D->d_avals) return -1;
+  *d_avals = D->d_avals;
+  *host_src = D->num_vals;
+  *gen = D->num_gen;
+  return 0;
 }
 
 extern "C" int tamd_dev_numeric(tamd_factor* F, const taucs_ccs_matrix* A)
@@ -927,6 +1032,53 @@ extern "C" int taucs_amd_get_numeric_timing(void* vL, taucs_amd_numeric_timing* 
 }
 
 // ---- solves ------------------------------------------------------------------
+
+template <typename T>
+static int emit_solve(tamd_dev* D, Emitter& E, int n, int nrhs, double* dX, int ldx,
+                      const double* dB, int ldb)
+{
+  T* W = (T*)D->W;
+  T* Y = (T*)D->Y;
+  T* Part = (T*)D->Part;
+  const T* L = (const T*)D->L;
+  const T* Iv = (const T*)D->Inv;
+  const FrontDesc* fr = D->d_fronts;
+  const int* rows = D->d_rows;
+  for (const Op& op : D->sv_ops) {
+    const PanelTask* tk = D->d_sv_tasks + op.start;
+    const dim3 grid(op.count);
+    switch (op.kind) {
+      case SV_INIT:
+        EMIT(0, true, k_solve_init<T>, grid, dim3(128), fr, rows, dB, ldb, W, nrhs);
+        break;
+      case SV_FSTEP:
+        EMIT(0, true, k_fwd_step<T>, grid, dim3(256), fr, tk, rows, L, Iv, W, Y, n, nrhs);
+        break;
+      case SV_FRECT:
+        EMIT(0, true, k_fwd_rect<T>, grid, dim3(256), fr, tk, rows, L, (const T*)Y, n, Part, nrhs);
+        break;
+      case SV_FRED:
+        EMIT(0, true, k_fwd_rect_reduce<T>, grid, dim3(256), fr, tk, (const T*)Part, W, nrhs);
+        break;
+      case SV_EXT:
+        EMIT(0, true, k_solve_ext<T>, grid, dim3(256), fr, tk, (const int*)D->d_rel, W, nrhs);
+        break;
+      case SV_GATHER:
+        EMIT(0, true, k_bwd_gather<T>, grid, dim3(256), fr, tk, rows, (const double*)dX, ldx, W, nrhs);
+        break;
+      case SV_BCOLS:
+        EMIT(0, true, k_bwd_cols<T>, grid, dim3(256), fr, tk, L, (const T*)W, Part, nrhs);
+        break;
+      case SV_BRED:
+        EMIT(0, true, k_bwd_cols_reduce<T>, grid, dim3(256), fr, tk, rows, (const T*)Part, Y, n, nrhs);
+        break;
+      case SV_BSTEP:
+        EMIT(0, true, k_bwd_step<T>, grid, dim3(256), fr, tk, rows, L, Iv, Y, n, dX, ldx, nrhs);
+        break;
+    }
+  }
+  return 0;
+}
 
 template <typename T>
 static int run_solve(tamd_factor* F, int nrhs, double* dX, int ldx, const double* dB, int ldb)
@@ -948,87 +1100,44 @@ static int run_solve(tamd_factor* F, int nrhs, double* dX, int ldx, const double
       return -1;
     D->W_rhs = cap;
   }
-  const bool use_graph = g_graphs != 0;
-  tamd_dev::SolveGraph* slot = nullptr;
-  std::unique_lock<std::mutex> cap_lk(g_mem_mu, std::defer_lock);
-  if (use_graph) {
-    for (auto& g : D->sv)
-      if (g.exec && g.nrhs == nrhs && g.prec == D->prec && g.x == dX && g.b == dB && g.ldx == ldx
-          && g.ldb == ldb) {
-        g.used = ++D->sv_clock;
-        HIPCHECK(hipGraphLaunch(g.exec, st));
-        return 0;
-      }
-    slot = &D->sv[0];
-    for (auto& g : D->sv)
-      if (g.used < slot->used) slot = &g;
-    if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
-    *slot = tamd_dev::SolveGraph();
-    cap_lk.lock();
-    HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  if (!g_graphs) {
+    Emitter E = eager_emitter(D);
+    if (emit_solve<T>(D, E, n, nrhs, dX, ldx, dB, ldb) != 0) return -1;
+    HIPCHECK(hipGetLastError());
+    return 0;
   }
-  T* W = (T*)D->W;
-  T* Y = (T*)D->Y;
-  T* Part = (T*)D->Part;
-  const T* L = (const T*)D->L;
-  const T* Iv = (const T*)D->Inv;
-  for (const Op& op : D->sv_ops) {
-    const PanelTask* tk = D->d_sv_tasks + op.start;
-    switch (op.kind) {
-      case SV_INIT:
-        hipLaunchKernelGGL(k_solve_init<T>, dim3(op.count), dim3(128), 0, st, D->d_fronts, D->d_rows,
-                           dB, ldb, W, nrhs);
-        break;
-      case SV_FSTEP:
-        hipLaunchKernelGGL(k_fwd_step<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk,
-                           D->d_rows, L, Iv, W, Y, n, nrhs);
-        break;
-      case SV_FRECT:
-        hipLaunchKernelGGL(k_fwd_rect<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk,
-                           D->d_rows, L, (const T*)Y, n, Part, nrhs);
-        break;
-      case SV_FRED:
-        hipLaunchKernelGGL(k_fwd_rect_reduce<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk,
-                           (const T*)Part, W, nrhs);
-        break;
-      case SV_EXT:
-        hipLaunchKernelGGL(k_solve_ext<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk, D->d_rel,
-                           W, nrhs);
-        break;
-      case SV_GATHER:
-        hipLaunchKernelGGL(k_bwd_gather<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk,
-                           D->d_rows, (const double*)dX, ldx, W, nrhs);
-        break;
-      case SV_BCOLS:
-        hipLaunchKernelGGL(k_bwd_cols<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk, L,
-                           (const T*)W, Part, nrhs);
-        break;
-      case SV_BRED:
-        hipLaunchKernelGGL(k_bwd_cols_reduce<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk,
-                           D->d_rows, (const T*)Part, Y, n, nrhs);
-        break;
-      case SV_BSTEP:
-        hipLaunchKernelGGL(k_bwd_step<T>, dim3(op.count), dim3(256), 0, st, D->d_fronts, tk, D->d_rows,
-                           L, Iv, Y, n, dX, ldx, nrhs);
-        break;
+  for (auto& g : D->sv)
+    if (g.exec && g.nrhs == nrhs && g.prec == D->prec && g.x == dX && g.b == dB && g.ldx == ldx
+        && g.ldb == ldb) {
+      g.used = ++D->sv_clock;
+      HIPCHECK(hipGraphLaunch(g.exec, st));
+      return 0;
     }
+  tamd_dev::SolveGraph* slot = &D->sv[0];
+  for (auto& g : D->sv)
+    if (g.used < slot->used) slot = &g;
+  if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+  *slot = tamd_dev::SolveGraph();
+  Emitter E;
+  E.main_st = st;
+  HIPCHECK(hipGraphCreate(&E.g, 0));
+  int rc = emit_solve<T>(D, E, n, nrhs, dX, ldx, dB, ldb);
+  hipError_t ie = rc == 0 ? hipGraphInstantiate(&slot->exec, E.g, nullptr, nullptr, 0)
+                          : hipErrorUnknown;
+  (void)hipGraphDestroy(E.g);
+  if (ie != hipSuccess) {
+    slot->exec = nullptr;
+    taucs_printf((char*)"taucs_amd: building the solve graph failed\n");
+    return -1;
   }
-  if (use_graph) {
-    hipGraph_t g = nullptr;
-    HIPCHECK(hipStreamEndCapture(st, &g));
-    hipError_t ie = hipGraphInstantiate(&slot->exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    HIPCHECK(ie);
-    slot->nrhs = nrhs;
-    slot->prec = D->prec;
-    slot->x = dX;
-    slot->b = dB;
-    slot->ldx = ldx;
-    slot->ldb = ldb;
-    slot->used = ++D->sv_clock;
-    HIPCHECK(hipGraphLaunch(slot->exec, st));
-  }
-  HIPCHECK(hipGetLastError());
+  slot->nrhs = nrhs;
+  slot->prec = D->prec;
+  slot->x = dX;
+  slot->b = dB;
+  slot->ldx = ldx;
+  slot->ldb = ldb;
+  slot->used = ++D->sv_clock;
+  HIPCHECK(hipGraphLaunch(slot->exec, st));
   return 0;
 }
 

@@ -125,12 +125,20 @@ __global__ void k_ratio(const double* num, const double* den, const int* done, d
 using namespace tamd;
 
 struct RefineCtx {
-  const taucs_ccs_matrix* A = nullptr;
-  const double* vals_ptr = nullptr;
-  int n = 0, nnz = 0;
+  // pattern identity (the CSR structure depends only on it)
+  const int* colptr = nullptr;
+  const int* rowind = nullptr;
+  int n = 0, nnz = 0;      // nnz: stored (lower) entries of A
   int* rp = nullptr;
   int* ci = nullptr;
+  int* src = nullptr;      // CSR slot -> index of the stored entry of A
   double* v = nullptr;
+  double* avals = nullptr; // device copy of A's values when they are not the factor's
+  // values identity: the device values the CSR copy was gathered from
+  const double* vals_from = nullptr;
+  unsigned long long gen = 0;
+  int nfull = 0;           // CSR entries
+  bool anorm_stale = true;
   double anorm_inf = 0.0;
   int cap = 0;  // rhs capacity of the work vectors
   double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *x = nullptr, *b = nullptr;
@@ -151,8 +159,8 @@ extern "C" void tamd_dev_free_ptr(void* p);
 static void ctx_free(RefineCtx* c)
 {
   if (!c) return;
-  void* ps[] = {c->rp, c->ci, c->v, c->r, c->z, c->p, c->q, c->x, c->b, c->partial, c->s1, c->s2, c->s3,
-                c->mask, c->rz, c->rz_new, c->scal, c->done};
+  void* ps[] = {c->rp, c->ci, c->src, c->v, c->avals, c->r, c->z, c->p, c->q, c->x, c->b, c->partial,
+                c->s1, c->s2, c->s3, c->mask, c->rz, c->rz_new, c->scal, c->done};
   for (void* p : ps) tamd_dev_free_ptr(p);
   delete c;
 }
@@ -175,15 +183,30 @@ extern "C" void tamd_refine_forget(void* key)
 static int dmalloc(void** p, size_t bytes) { return tamd_dev_malloc(p, bytes); }
 
 // Build (or reuse) the full-pattern CSR of A on the device.
-static RefineCtx* get_ctx(void* key, const taucs_ccs_matrix* A, int nrhs)
+extern "C" int tamd_dev_values_of(tamd_factor* F, const double** d_avals, const double** host_src,
+                                  unsigned long long* gen);
+
+// v[e] = vals[src[e]]
+__global__ void k_gather_vals(int nnz, const int* __restrict__ src, const double* __restrict__ vals,
+                              double* __restrict__ v)
 {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nnz) v[e] = vals[src[e]];
+}
+
+// The full-pattern CSR of A (built once per pattern) and its values (gathered
+// on the device whenever the factor was refactored or A's values changed).
+static RefineCtx* get_ctx(tamd_factor* F, const taucs_ccs_matrix* A, int nrhs, hipStream_t st)
+{
+  void* key = F;
   RefineCtx* c = nullptr;
   {
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     for (auto& e : g_ctx)
       if (e.key == key) c = e.ctx;
   }
-  if (c && (c->A != A || c->vals_ptr != A->values.d || c->n != A->n || c->nnz != A->colptr[A->n])) {
+  if (c && (c->colptr != A->colptr || c->rowind != A->rowind || c->n != A->n
+            || c->nnz != A->colptr[A->n])) {
     tamd_refine_forget(key);
     c = nullptr;
   }
@@ -199,35 +222,54 @@ static RefineCtx* get_ctx(void* key, const taucs_ccs_matrix* A, int nrhs)
       }
     for (int i = 0; i < n; i++) cnt[i + 1] += cnt[i];
     int nnz = cnt[n];
-    std::vector<int> fill(cnt.begin(), cnt.end() - 1), ci(nnz);
-    std::vector<double> v(nnz);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1), ci(nnz), src(nnz);
     for (int j = 0; j < n; j++)
       for (int k = A->colptr[j]; k < A->colptr[j + 1]; k++) {
         int i = A->rowind[k];
-        double a = A->values.d[k];
-        ci[fill[i]] = j; v[fill[i]++] = a;
-        if (i != j) { ci[fill[j]] = i; v[fill[j]++] = a; }
+        ci[fill[i]] = j; src[fill[i]++] = k;
+        if (i != j) { ci[fill[j]] = i; src[fill[j]++] = k; }
       }
-    double an = 0.0;
-    for (int i = 0; i < n; i++) {
-      double s = 0.0;
-      for (int k = cnt[i]; k < cnt[i + 1]; k++) s += fabs(v[k]);
-      an = std::max(an, s);
-    }
-    c->A = A; c->vals_ptr = A->values.d; c->n = n; c->nnz = nl; c->anorm_inf = an;
+    c->colptr = A->colptr; c->rowind = A->rowind; c->n = n; c->nnz = nl;
+    c->nfull = nnz;
     if (dmalloc((void**)&c->rp, sizeof(int) * (n + 1)) || dmalloc((void**)&c->ci, sizeof(int) * nnz)
-        || dmalloc((void**)&c->v, sizeof(double) * nnz)) {
+        || dmalloc((void**)&c->src, sizeof(int) * nnz) || dmalloc((void**)&c->v, sizeof(double) * nnz)) {
       ctx_free(c);
       return nullptr;
     }
-    if (hipMemcpy(c->rp, cnt.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice) != hipSuccess
-        || hipMemcpy(c->ci, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice) != hipSuccess
-        || hipMemcpy(c->v, v.data(), sizeof(double) * nnz, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpyAsync(c->rp, cnt.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, st) != hipSuccess
+        || hipMemcpyAsync(c->ci, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, st) != hipSuccess
+        || hipMemcpyAsync(c->src, src.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, st) != hipSuccess
+        || hipStreamSynchronize(st) != hipSuccess) {
       ctx_free(c);
       return nullptr;
     }
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     g_ctx.push_back({key, c});
+  }
+  // values: the factor's own device copy when A is the matrix it was factored
+  // from, else A's host values uploaded now
+  const double* dvals = nullptr;
+  const double* hsrc = nullptr;
+  unsigned long long gen = 0;
+  if (tamd_dev_values_of(F, &dvals, &hsrc, &gen) != 0) return nullptr;
+  const double* from;
+  if (hsrc == A->values.d) {
+    from = dvals;
+  } else {
+    if (!c->avals && dmalloc((void**)&c->avals, sizeof(double) * (size_t)(c->nnz > 0 ? c->nnz : 1)))
+      return nullptr;
+    if (hipMemcpyAsync(c->avals, A->values.d, sizeof(double) * (size_t)c->nnz, hipMemcpyHostToDevice,
+                       st) != hipSuccess)
+      return nullptr;
+    from = c->avals;
+    gen = ~0ULL;  // always refresh
+  }
+  if (from != c->vals_from || gen != c->gen || gen == ~0ULL) {
+    hipLaunchKernelGGL(k_gather_vals, dim3((c->nfull + 255) / 256), dim3(256), 0, st, c->nfull, c->src,
+                       from, c->v);
+    c->vals_from = from;
+    c->gen = gen;
+    c->anorm_stale = true;
   }
   if (c->cap < nrhs) {
     void** vs[] = {(void**)&c->r, (void**)&c->z, (void**)&c->p, (void**)&c->q, (void**)&c->x, (void**)&c->b};
@@ -305,9 +347,16 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
   if (!F || !F->has_numeric || !A || A->n != F->n || nrhs <= 0) return -1;
   hipStream_t st;
   if (tamd_dev_stream_of(F, &st) != 0) return -1;
-  RefineCtx* c = get_ctx(vL, A, nrhs);
+  RefineCtx* c = get_ctx(F, A, nrhs, st);
   if (!c) return -1;
   int n = c->n;
+  if (c->anorm_stale) {  // |A|_inf = max row sum of |a_ij|, on the device
+    hipLaunchKernelGGL(k_csr_absrow, dim3((n + 255) / 256), dim3(256), 0, st, n, c->rp, c->v, c->r);
+    colreduce(st, c, c->r, nullptr, 1, c->s1, 1);
+    HIPOK(hipMemcpyAsync(&c->anorm_inf, c->s1, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    c->anorm_stale = false;
+  }
   if (maxit <= 0) maxit = 100;
   HIPOK(hipMemcpy2DAsync(c->b, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
                          hipMemcpyHostToDevice, st));

@@ -30,7 +30,7 @@ typedef struct {
   tamd_factor** sys;      /* per-system symbolic factors (no device state) */
   long long*    xoff;     /* [count+1] */
   tamd_factor   all;      /* the merged device schedule; n = n_total */
-  double*       vals;     /* concatenated A values */
+  double*       vals;     /* concatenated A values (the block-diagonal matrix's) */
   taucs_ccs_matrix* blockdiag; /* concatenated pattern for refinement */
 } tamd_batch;
 
@@ -43,8 +43,7 @@ static void batch_free(tamd_batch* B)
     if (B->sys && B->sys[q]) taucs_supernodal_factor_free(B->sys[q]);
   tamd_free(B->sys);
   tamd_free(B->xoff);
-  tamd_free(B->vals);
-  taucs_ccs_free(B->blockdiag);
+  taucs_ccs_free(B->blockdiag);   /* owns vals */
   tamd_free(B);
 }
 
@@ -62,7 +61,6 @@ static int gather_values(tamd_batch* B, taucs_ccs_matrix** A)
       return -1;
     }
     memcpy(B->vals + off, A[q]->values.d, sizeof(double) * (size_t)nnz);
-    memcpy(B->blockdiag->values.d + off, A[q]->values.d, sizeof(double) * (size_t)nnz);
     off += nnz;
   }
   return 0;
@@ -92,10 +90,11 @@ void* taucs_amd_batch_factor_llt_mf(int count, taucs_ccs_matrix** A, int precisi
   B->n_total = (int)n;
   B->nnz_total = nnz;
 
-  /* block-diagonal copy of the pattern (for the FP64 residuals) */
+  /* block-diagonal matrix; its value array is the concatenated values the
+     device factors from, so refinement against it uses the device copy */
   B->blockdiag = taucs_ccs_create((int)n, (int)n, (int)nnz, A[0]->flags);
-  B->vals = (double*)tamd_malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
-  if (!B->blockdiag || !B->vals) { batch_free(B); return NULL; }
+  if (!B->blockdiag) { batch_free(B); return NULL; }
+  B->vals = B->blockdiag->values.d;
   {
     long long off = 0;
     int j;

@@ -122,3 +122,28 @@ def test_lookahead_gives_identical_results():
             F1.free()
         finally:
             T.lib.taucs_amd_set_graphs(1)
+
+
+def test_refinement_follows_refactorization():
+    """The refinement context keeps its CSR structure across numeric
+    refactorizations and takes the new values from the factor's device copy."""
+    print("case: refine after refactorization with new values", flush=True)
+    A = T.mesh3d(14)
+    perm, inv = T.order(A, "genmmd")
+    PA = T.permute(A, perm, inv)
+    n, cp, ri, v = PA.arrays()
+    b = np.stack([T.random_uniform(n, 5), T.random_uniform(n, 6)], 1)
+    F = T.factor_symbolic(PA)
+    assert F.numeric(PA, T.FP32) == 0
+    x1, i1 = F.solve_refined(PA, b, method=0, tol=1e-14, maxit=50)
+    assert i1.converged
+    B2 = T.from_arrays(n, cp, ri, 3.0 * v)
+    assert F.numeric(B2, T.FP32) == 0
+    x2, i2 = F.solve_refined(B2, b, method=0, tol=1e-14, maxit=50)
+    assert i2.converged
+    # both at backward error <= 1e-14: they agree to about cond(A) * 1e-14
+    assert np.allclose(x2, x1 / 3.0, rtol=1e-9, atol=1e-12)
+    # refining against a matrix that is not the factored one uploads its values
+    x3, i3 = F.solve_refined(PA, b, method=1, tol=1e-14, maxit=100)
+    assert i3.converged
+    assert np.allclose(x3, x1, rtol=1e-9, atol=1e-12)
