@@ -24,6 +24,7 @@ template <typename T> __global__ void k_front_small(const FrontDesc*, const int*
 template <typename T> __global__ void k_potrf_diag(const FrontDesc*, const PanelTask*, T*, T*, int*);
 template <typename T> __global__ void k_potrf_wave(const FrontDesc*, const PanelTask*, T*, T*, int*);
 template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*, const T*);
+template <typename T> __global__ void k_trsm_tile(const GemmTask*, const T*, T*, T*, const T*);
 __global__ void k_gemm_tile_bf16(const GemmTask*, const float*, float*, float*);
 template <typename T> __global__ void k_extend_add(const FrontDesc*, const EATask*, const int*, T*, T*);
 template <typename T> __global__ void k_solve_init(const FrontDesc*, const int*, const double*, int, T*, int);
@@ -859,7 +860,7 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
                (const PanelTask*)(D->d_potrf + op.start), L, Iv, D->d_info);
         break;
       case TAMD_OP_TRSM:
-        EMIT(lane, join, k_gemm_tile<T>, dim3(op.count), dim3(256),
+        EMIT(lane, join, k_trsm_tile<T>, dim3(op.count), dim3(256),
              (const GemmTask*)(D->d_trsm + op.start), (const T*)L, L, U, (const T*)Iv);
         break;
       case TAMD_OP_GEMM:

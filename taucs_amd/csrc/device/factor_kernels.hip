@@ -202,18 +202,20 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const FrontDesc* __restrict_
 }
 
 // ---- one-wave Cholesky + inverse of a 64 x 64 diagonal block ----------------
-// Lane i holds row i of the block in 64 registers; every value another lane
-// needs is broadcast with v_readlane (the loops are fully unrolled, so every
-// register index and source lane is a constant), so the whole block is
-// factored and inverted without LDS and without a barrier.
-//   Cholesky (right-looking, src/taucs_sn_llt.c:1112-1120 calls dpotrf on
-//   the same block): for k: d = sqrt(a_kk); rows i > k scale a_ik by 1/d;
-//   a_ij -= a_ik a_jk for k < j <= i.
-//   Inverse in place, row sweep: after step t row t of X = L^{-1} is final:
-//   x_tc = x_tc / l_tt (c < t), x_tt = 1 / l_tt; rows i > t then take
-//   x_ic -= l_it x_tc (c <= t), reusing l_ic's register once l_ic is spent.
+// Lane i holds row i of the block in 64 registers (loops fully unrolled, so
+// every register index and every source lane is a constant); values other
+// lanes need are broadcast with v_readlane.  No LDS, no barrier.  Updates
+// are applied to all lanes unmasked: entries above the diagonal take garbage
+// and are never read or stored (per-lane masks in the unrolled code would
+// cost a mask register per column).
+//   Cholesky, right-looking (src/taucs_sn_llt.c:1112-1120 calls dpotrf on
+//   the same block): d = sqrt(a_kk), l_ik = a_ik / d, a_ij -= l_ik l_jk (lane
+//   j's l_jk is read after lane j has scaled it).
+//   Inverse X = L^{-1} in place, row sweep: before step s, lane s holds the
+//   unscaled row s of X (p_sc, c < s); lanes i > s take
+//   x_ic -= l_is p_sc / l_ss, and the rows are scaled by 1/l_ii at the end.
 // A kb < 64 block is padded with the identity, whose factor and inverse are
-// the identity, so the same fully unrolled code serves every block size.
+// the identity.
 template <typename T>
 __device__ __forceinline__ T rlane(T v, int l);
 template <>
@@ -228,7 +230,6 @@ __device__ __forceinline__ float rlane<float>(float v, int l)
 {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
-
 template <typename T>
 __global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__ fr,
                                                    const PanelTask* __restrict__ tasks,
@@ -237,45 +238,51 @@ __global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__
 {
   const PanelTask t = tasks[blockIdx.x];
   const FrontDesc d = fr[t.front];
-  const int m = d.m, kb = t.kb, lane = threadIdx.x;
+  const long long m = d.m;
+  const int kb = t.kb, lane = threadIdx.x;
   T* gp = L + d.offL + t.k0 + (long long)t.k0 * m;
   T r[64];
 #pragma unroll
   for (int c = 0; c < 64; c++)
-    r[c] = (lane < kb && c < kb) ? gp[lane + (long long)c * m] : (lane == c ? (T)1 : (T)0);
+    r[c] = (lane < kb && c < kb) ? gp[lane + c * m] : (lane == c ? (T)1 : (T)0);
   bool bad = false;
+  T myrd = (T)1;   // 1 / l_ii of this lane's row
 #pragma unroll
   for (int k = 0; k < 64; k++) {
     T p = rlane(r[k], k);
     if (!(p > (T)0)) { bad = true; p = (T)1; }
     const T dk = sqrt(p), rdk = (T)1 / dk;
-    r[k] = lane == k ? dk : (lane > k ? r[k] * rdk : r[k]);
+    const T lik = r[k] * rdk;
+    r[k] = lane == k ? dk : lik;
+    myrd = lane == k ? rdk : myrd;
 #pragma unroll
-    for (int j = k + 1; j < 64; j++) {
-      const T ljk = rlane(r[k], j);
-      if (lane >= j) r[j] -= r[k] * ljk;
-    }
+    for (int j = k + 1; j < 64; j++) r[j] -= lik * rlane(r[k], j);  // l_ik l_jk
   }
+  T* gw = gp;  // a fresh copy, so the loads' addresses need not stay live
+  asm volatile("" : "+s"(gw));
 #pragma unroll
   for (int c = 0; c < 64; c++)
-    if (lane < kb && c < kb && lane >= c) gp[lane + (long long)c * m] = r[c];
+    if (lane < kb && c < kb && lane >= c) gw[lane + c * m] = r[c];
+  // per-phase copies of the lane index keep the compiler from sharing (and
+  // keeping live) one phase's 64 lane comparisons in the next
+  int ln = lane;
+  asm volatile("" : "+v"(ln));
 #pragma unroll
   for (int s = 0; s < 64; s++) {
-    // row s becomes final (all lanes stay active around the broadcasts)
-    const T rls = (T)1 / rlane(r[s], s);
+    const T rls = rlane(myrd, s);
+    const T lis = ln > s ? r[s] : (T)0;
+    r[s] = -lis * rls;
 #pragma unroll
-    for (int c = 0; c < s; c++) r[c] = lane == s ? r[c] * rls : r[c];
-    const T lis = lane > s ? r[s] : (T)0;
-    r[s] = lane == s ? rls : (lane > s ? (T)0 : r[s]);
-#pragma unroll
-    for (int c = 0; c <= s; c++) {
-      const T xsc = rlane(r[c], s);
-      r[c] -= lis * xsc;
-    }
+    for (int c = 0; c < s; c++) r[c] -= lis * (rlane(r[c], s) * rls);
   }
-  T* iv = Inv + d.offI + (long long)(t.k0 / 64) * 4096;
+  asm volatile("" : "+v"(ln));
 #pragma unroll
-  for (int c = 0; c < 64; c++) iv[lane * 64 + c] = (lane < kb && c < kb && c <= lane) ? r[c] : (T)0;
+  for (int c = 0; c < 64; c++) r[c] = c == ln ? myrd : r[c] * myrd;
+  T* iv = Inv + d.offI + (long long)(t.k0 / 64) * 4096;
+  asm volatile("" : "+s"(iv));
+  asm volatile("" : "+v"(ln));
+#pragma unroll
+  for (int c = 0; c < 64; c++) iv[ln * 64 + c] = (ln < kb && c < kb && c <= ln) ? r[c] : (T)0;
   if (lane == 0 && bad) atomicOr(info, 1);
 }
 
@@ -313,9 +320,8 @@ template <> struct Mfma<float> {
 // K is consumed in chunks of 32: the next chunk is fetched from global memory
 // into registers while the MFMAs run on the current chunk in LDS.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ tasks,
-                                                   const T* Lsrc, T* L, T* U,
-                                                   const T* __restrict__ Inv)
+__device__ __forceinline__ void gemm_tile(const GemmTask* __restrict__ tasks, const T* Lsrc, T* L,
+                                          T* U, const T* __restrict__ Inv)
 {
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T As[TK][64 + TPAD];
@@ -376,6 +382,24 @@ __global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ 
           else C[i + (long long)j * t.ldc] -= acc[a][b][r];
         }
       }
+}
+
+// Trailing update / SYRK tiles (C -= A B^T) and the panel solves (C = A
+// inv(L_kk)^T, flag 4) share the tile code; two symbols keep their launches
+// apart in profiles.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ tasks,
+                                                   const T* Lsrc, T* L, T* U,
+                                                   const T* __restrict__ Inv)
+{
+  gemm_tile<T>(tasks, Lsrc, L, U, Inv);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_tile(const GemmTask* __restrict__ tasks,
+                                                   const T* Lsrc, T* L, T* U,
+                                                   const T* __restrict__ Inv)
+{
+  gemm_tile<T>(tasks, Lsrc, L, U, Inv);
 }
 
 // Same tile, BF16 inputs with FP32 accumulation (v_mfma_f32_16x16x32_bf16):
@@ -468,6 +492,7 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
   template __global__ void k_potrf_diag<T>(const FrontDesc*, const PanelTask*, T*, T*, int*); \
   template __global__ void k_potrf_wave<T>(const FrontDesc*, const PanelTask*, T*, T*, int*); \
   template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
+  template __global__ void k_trsm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*);
 TAMD_INST(double)
 TAMD_INST(float)

@@ -64,6 +64,28 @@ __device__ __forceinline__ void load_inv(T* It, const T* iv)
   }
 }
 
+// The same two loads issued together: all 32 global loads of a thread are in
+// flight before the first LDS store (the step kernels are latency-bound).
+template <typename T>
+__device__ __forceinline__ void load_inv_and_tile(T* It, const T* iv, T* Lt, const T* src,
+                                                  long long ld, int nr, int nc)
+{
+  T a[16], b[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int e = threadIdx.x + 256 * u;
+    a[u] = iv[e];
+    const int i = e & 63, j = e >> 6;
+    b[u] = (i < nr && j < nc) ? src[i + (long long)j * ld] : (T)0;
+  }
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int e = threadIdx.x + 256 * u;
+    It[(e >> 6) + (e & 63) * 65] = a[u];
+    Lt[(e & 63) + (e >> 6) * 65] = b[u];
+  }
+}
+
 // One 64-column block step of the forward solve.  Every task forms
 // y_blk = inv(L_kk) w_blk from the block's rows of W_f (read-only in this
 // step) and updates its 64 rows of the diagonal block's column range below
@@ -78,6 +100,7 @@ __global__ __launch_bounds__(256) void k_fwd_step(const FrontDesc* __restrict__ 
                                                   const T* __restrict__ Inv, T* __restrict__ W,
                                                   T* __restrict__ Y, int ldy, int nrhs)
 {
+  __shared__ T It[64 * 65];
   __shared__ T Lt[64 * 65];
   __shared__ T Wb[64 * SMAXR];
   __shared__ T Yb[64 * SMAXR];
@@ -85,7 +108,8 @@ __global__ __launch_bounds__(256) void k_fwd_step(const FrontDesc* __restrict__ 
   const FrontDesc d = fr[t.front];
   const int m = d.m, kb = t.kb, tid = threadIdx.x;
   T* w = W + d.offW * nrhs;
-  load_inv(Lt, Inv + d.offI + (long long)(t.k0 / 64) * 4096);
+  load_inv_and_tile(It, Inv + d.offI + (long long)(t.k0 / 64) * 4096, Lt,
+                    L + d.offL + t.r0 + (long long)t.k0 * m, m, t.nr, kb);
   for (int e = tid; e < 64 * nrhs; e += 256) {
     int i = e & 63, r = e >> 6;
     Wb[i + r * 64] = i < kb ? w[t.k0 + i + r * m] : (T)0;
@@ -94,7 +118,7 @@ __global__ __launch_bounds__(256) void k_fwd_step(const FrontDesc* __restrict__ 
   for (int e = tid; e < 64 * nrhs; e += 256) {
     int i = e & 63, r = e >> 6;
     T acc = (T)0;
-    for (int q = 0; q <= i; q++) acc += Lt[i + q * 65] * Wb[q + r * 64];
+    for (int q = 0; q <= i; q++) acc += It[i + q * 65] * Wb[q + r * 64];
     Yb[i + r * 64] = acc;
   }
   __syncthreads();
@@ -106,8 +130,6 @@ __global__ __launch_bounds__(256) void k_fwd_step(const FrontDesc* __restrict__ 
     }
   }
   if (t.nr <= 0) return;
-  load_tile(Lt, L + d.offL + t.r0 + (long long)t.k0 * m, m, t.nr, kb);
-  __syncthreads();
   const int i = tid & 63, g = tid >> 6;
   if (i < t.nr) {
     for (int q = 0; q < 4; q++) {
@@ -289,6 +311,7 @@ __global__ __launch_bounds__(256) void k_bwd_step(const FrontDesc* __restrict__ 
                                                   int ldy, double* __restrict__ x, int ldx,
                                                   int nrhs)
 {
+  __shared__ T It[64 * 65];
   __shared__ T Lt[64 * 65];
   __shared__ T Zb[64 * SMAXR];
   __shared__ T Xb[64 * SMAXR];
@@ -296,7 +319,8 @@ __global__ __launch_bounds__(256) void k_bwd_step(const FrontDesc* __restrict__ 
   const FrontDesc d = fr[t.front];
   const int m = d.m, kb = t.kb, tid = threadIdx.x;
   const int* rw = rows + d.rows;
-  load_inv(Lt, Inv + d.offI + (long long)(t.k0 / 64) * 4096);
+  load_inv_and_tile(It, Inv + d.offI + (long long)(t.k0 / 64) * 4096, Lt,
+                    L + d.offL + t.k0 + (long long)t.r0 * m, m, kb, t.nr);
   for (int e = tid; e < 64 * nrhs; e += 256) {
     int i = e & 63, r = e >> 6;
     Zb[i + r * 64] = i < kb ? Y[rw[t.k0 + i] + (long long)r * ldy] : (T)0;
@@ -305,7 +329,7 @@ __global__ __launch_bounds__(256) void k_bwd_step(const FrontDesc* __restrict__ 
   for (int e = tid; e < 64 * nrhs; e += 256) {
     int i = e & 63, r = e >> 6;
     T acc = (T)0;
-    for (int q = i; q < kb; q++) acc += Lt[q + i * 65] * Zb[q + r * 64];
+    for (int q = i; q < kb; q++) acc += It[q + i * 65] * Zb[q + r * 64];
     Xb[i + r * 64] = acc;
   }
   __syncthreads();
@@ -316,8 +340,6 @@ __global__ __launch_bounds__(256) void k_bwd_step(const FrontDesc* __restrict__ 
     }
   }
   if (t.nr <= 0) return;
-  load_tile(Lt, L + d.offL + t.k0 + (long long)t.r0 * m, m, kb, t.nr);
-  __syncthreads();
   const int j = tid & 63, g = tid >> 6;
   if (j < t.nr) {
     const int gj = rw[t.r0 + j];
