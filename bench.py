@@ -294,6 +294,24 @@ def main():
     step(F, T.FP64)
     T.lib.taucs_amd_set_profiling(0)
     tm = F.timing()
+    prof = F.op_profile()
+    kinds = {1: "assemble", 2: "small_fronts", 3: "potrf", 4: "trsm", 5: "gemm_syrk", 6: "extend_add",
+             7: "zero_update_blocks"}
+    breakdown = {}
+    for kd, name in kinds.items():
+        sel = prof["kind"] == kd
+        if not sel.any():
+            continue
+        ms = float(prof["ms"][sel].sum())
+        e = {"launches": int(sel.sum()), "ms": round(ms, 3)}
+        if prof["flops"][sel].sum() > 0:
+            e["flops"] = float(prof["flops"][sel].sum())
+            e["tflops"] = round(e["flops"] / (ms * 1e-3) / 1e12, 3) if ms > 0 else None
+        if prof["bytes"][sel].sum() > 0:
+            e["bytes"] = float(prof["bytes"][sel].sum())
+            e["gbps"] = round(e["bytes"] / (ms * 1e-3) / 1e9, 1) if ms > 0 else None
+        breakdown[name] = e
+    log(f"breakdown: {breakdown}")
     gemm_avg_ms = tm.gemm_ms / max(tm.gemm_launches, 1)
     gemm_rate = tm.gemm_flops / (tm.gemm_ms * 1e-3) if tm.gemm_ms > 0 else 0.0
     log(f"profile: total {tm.total_ms:.2f} ms, front {tm.front_ms:.2f} ms, gemm {tm.gemm_ms:.2f} ms "
@@ -373,6 +391,7 @@ def main():
                      "avg_launch_ms": round(gemm_avg_ms, 4), "launches": tm.gemm_launches,
                      "traffic": None},
         "cpu_baseline": cpu,
+        "numeric_breakdown": breakdown,
         "mixed_precision_varcoef": varcoef,
         "batched": batch,
         "serving": serving,

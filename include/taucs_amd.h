@@ -153,6 +153,13 @@ typedef struct {
 void taucs_amd_set_profiling(int on);
 int  taucs_amd_get_numeric_timing(void* L, taucs_amd_numeric_timing* t);
 int  taucs_amd_batch_get_timing(void* B, taucs_amd_numeric_timing* t);
+/* Per-op record of the numeric schedule (ops in launch order): kind
+   (1 assemble, 2 small fronts, 3 POTRF, 4 TRSM, 5 GEMM/SYRK, 6 extend-add,
+   7 zero), tree level, task count, algorithmic flops and bytes, and the
+   device time (ms) of the last factorization run with profiling on.
+   Returns the number of ops (arrays hold up to cap entries). */
+int  taucs_amd_get_op_profile(void* L, int cap, int* kind, int* level, int* count,
+                              double* flops, double* bytes, double* ms);
 
 #ifdef __cplusplus
 }

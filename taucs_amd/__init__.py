@@ -98,6 +98,7 @@ _sigs = {
     "taucs_amd_set_graphs": (None, [C.c_int]),
     "taucs_amd_set_lookahead": (None, [C.c_int]),
     "taucs_amd_set_potrf_kernel": (None, [C.c_int]),
+    "taucs_amd_get_op_profile": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 6),
     "taucs_amd_batch_factor_llt_mf": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int]),
     "taucs_amd_batch_numeric": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "taucs_amd_batch_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
@@ -339,6 +340,17 @@ class Factor:
         t = NumericTiming()
         lib.taucs_amd_get_numeric_timing(self.h, C.byref(t))
         return t
+
+    def op_profile(self):
+        """Per-op kind, level, tasks, flops, bytes and ms of the last profiled numeric run."""
+        n = lib.taucs_amd_get_op_profile(self.h, 0, None, None, None, None, None, None)
+        if n < 0:
+            raise RuntimeError("no device schedule")
+        k, lv, c = (np.zeros(n, dtype=np.int32) for _ in range(3))
+        f, b, ms = (np.zeros(n) for _ in range(3))
+        lib.taucs_amd_get_op_profile(self.h, n, _dptr(k), _dptr(lv), _dptr(c), _dptr(f), _dptr(b),
+                                     _dptr(ms))
+        return {"kind": k, "level": lv, "count": c, "flops": f, "bytes": b, "ms": ms}
 
     def free(self):
         if self.h:

@@ -23,6 +23,7 @@ template <typename T> __global__ void k_assemble(int, const int*, const long lon
 template <typename T> __global__ void k_front_small(const FrontDesc*, const int*, T*, T*, T*, int*);
 template <typename T> __global__ void k_potrf_diag(const FrontDesc*, const PanelTask*, T*, T*, int*);
 template <typename T> __global__ void k_potrf_wave(const FrontDesc*, const PanelTask*, T*, T*, int*);
+template <typename T> __global__ void k_front_wave(const FrontDesc*, const int*, T*, T*, T*, int*);
 template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*, const T*);
 template <typename T> __global__ void k_trsm_tile(const GemmTask*, const T*, T*, T*, const T*);
 __global__ void k_gemm_tile_bf16(const GemmTask*, const float*, float*, float*);
@@ -84,6 +85,8 @@ struct tamd_dev {
   std::vector<Op> sv_ops;
   int* d_info = nullptr;
   std::vector<Op> ops;
+  std::vector<double> op_flops, op_bytes;  // algorithmic work of each op
+  std::vector<float> op_ms;                // device time of each op (profiling runs)
   std::vector<int> lvl_ptr;      // fronts of level l: [lvl_ptr[l], lvl_ptr[l+1])
   int n_fronts = 0, ncol = 0, nnz = 0, n_levels = 0;
   long long W_per_rhs = 0;
@@ -610,6 +613,32 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     }
   }
   D->ncol = (int)ncol;
+  // algorithmic work per op: tile flops (lower half of diagonal tiles), and
+  // extend-add / assembly bytes (read child + read-modify-write parent)
+  D->op_flops.assign(D->ops.size(), 0.0);
+  D->op_bytes.assign(D->ops.size(), 0.0);
+  for (size_t o = 0; o < D->ops.size(); o++) {
+    const Op& op = D->ops[o];
+    double f = 0, b = 0;
+    if (op.kind == TAMD_OP_GEMM)
+      for (int q = op.start; q < op.start + op.count; q++) f += tile_flops(gemm[q]);
+    else if (op.kind == TAMD_OP_TRSM)
+      for (int q = op.start; q < op.start + op.count; q++)
+        f += 2.0 * trsm[q].M * trsm[q].N * trsm[q].K;
+    else if (op.kind == TAMD_OP_POTRF)
+      f = op.count * (2.0 / 3.0) * 64.0 * 64.0 * 64.0;
+    else if (op.kind == TAMD_OP_EXTEND)
+      for (int q = op.start; q < op.start + op.count; q++) {
+        const int uc = fr[ea[q].child].m - fr[ea[q].child].s;
+        for (int j = ea[q].j0; j < ea[q].j1; j++) b += 24.0 * (uc - j);
+      }
+    else if (op.kind == TAMD_OP_ASSEMBLE)
+      b = 20.0 * (double)nnz_tot;
+    else if (op.kind == TAMD_OP_ZERO)
+      for (int q = op.start; q < op.start + op.count; q++) b += 8.0 * (double)zero[q].len;
+    D->op_flops[o] = f;
+    D->op_bytes[o] = b;
+  }
   D->struct_bytes = sizeof(FrontDesc) * fr.size() + sizeof(int) * (rows.size() + rel.size())
                     + sizeof(long long) * adest.size()
                     + sizeof(int) * (small.size() + colptr.size() + 1)
@@ -828,6 +857,7 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
   if (prof) {
     HIPCHECK(hipEventCreate(&a0));
     HIPCHECK(hipEventCreate(&a1));
+    D->op_ms.assign(D->ops.size(), 0.0f);
   }
   EMIT(0, true, k_fill<T>, dim3(grid_for(D->L_elems, 256)), dim3(256), L, D->L_elems);
   EMIT(0, true, k_fill<int>, dim3(1), dim3(64), D->d_info, 1LL);
@@ -848,8 +878,12 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
              (const int*)D->d_colptr, (const long long*)D->d_adest, (const double*)D->d_avals, L);
         break;
       case TAMD_OP_SMALL:
-        EMIT(lane, join, k_front_small<T>, dim3(op.count), dim3(256), (const FrontDesc*)D->d_fronts,
-             (const int*)(D->d_small + op.start), L, U, Iv, D->d_info);
+        if (g_potrf_wave)
+          EMIT(lane, join, k_front_wave<T>, dim3(op.count), dim3(64), (const FrontDesc*)D->d_fronts,
+               (const int*)(D->d_small + op.start), L, U, Iv, D->d_info);
+        else
+          EMIT(lane, join, k_front_small<T>, dim3(op.count), dim3(256), (const FrontDesc*)D->d_fronts,
+               (const int*)(D->d_small + op.start), L, U, Iv, D->d_info);
         break;
       case TAMD_OP_POTRF:
         if (g_potrf_wave)
@@ -881,6 +915,7 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
       HIPCHECK(hipEventRecord(a1, E.main_st));
       HIPCHECK(hipEventSynchronize(a1));
       HIPCHECK(hipEventElapsedTime(&ms, a0, a1));
+      D->op_ms[&op - D->ops.data()] = ms;
       if (op.kind == TAMD_OP_ASSEMBLE || op.kind == TAMD_OP_EXTEND || op.kind == TAMD_OP_ZERO)
         D->timing.assembly_ms += ms;
       else
@@ -1022,6 +1057,26 @@ extern "C" int tamd_dev_values_of(tamd_factor* F, const double** d_avals, const 
 extern "C" int tamd_dev_numeric(tamd_factor* F, const taucs_ccs_matrix* A)
 {
   return tamd_dev_numeric_values(F, A->values.d);
+}
+
+// Per-op record of the numeric schedule: kind (TAMD_OP_*), level, tasks,
+// algorithmic flops and bytes, and the device time of the last profiled run.
+extern "C" int taucs_amd_get_op_profile(void* vL, int cap, int* kind, int* level, int* count,
+                                        double* flops, double* bytes, double* ms)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  if (!F || !F->dev) return -1;
+  tamd_dev* D = F->dev;
+  const int n = (int)D->ops.size();
+  for (int o = 0; o < n && o < cap; o++) {
+    if (kind) kind[o] = D->ops[o].kind;
+    if (level) level[o] = D->ops[o].level;
+    if (count) count[o] = D->ops[o].count;
+    if (flops) flops[o] = D->op_flops[o];
+    if (bytes) bytes[o] = D->op_bytes[o];
+    if (ms) ms[o] = o < (int)D->op_ms.size() ? D->op_ms[o] : 0.0;
+  }
+  return n;
 }
 
 extern "C" int taucs_amd_get_numeric_timing(void* vL, taucs_amd_numeric_timing* t)

@@ -286,6 +286,86 @@ __global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__
   if (lane == 0 && bad) atomicOr(info, 1);
 }
 
+// Whole front with m <= 64 in one wavefront, the same register scheme as
+// k_potrf_wave: lane i holds row i of the m x m front [F1 .; F2 U] (panel
+// columns from L, update block from U, which already holds the children's
+// extend-adds).  The first s right-looking steps leave L in the panel columns
+// and the Schur complement U - F2 F2^T (the reference's dsyrk,
+// src/taucs_sn_llt.c:1178-1186) in the update block; then the s x s
+// diagonal block is inverted in place.  Rows and columns beyond m are
+// identity padding.
+template <typename T>
+__global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__ fr,
+                                                   const int* __restrict__ list,
+                                                   T* __restrict__ L, T* __restrict__ U,
+                                                   T* __restrict__ Inv, int* __restrict__ info)
+{
+  const FrontDesc d = fr[list[blockIdx.x]];
+  const int s = d.s, m = d.m, u = m - s, lane = threadIdx.x;
+  T* gp = L + d.offL;
+  T* gu = U + d.offU;
+  T r[64];
+#pragma unroll
+  for (int c = 0; c < 64; c++) {
+    T v = lane == c ? (T)1 : (T)0;
+    if (c < s) {
+      if (lane < m) v = gp[lane + c * m];
+    } else if (c < m) {
+      if (lane >= s && lane < m) v = gu[(lane - s) + (c - s) * u];
+    }
+    r[c] = v;
+  }
+  bool bad = false;
+  T myrd = (T)1;
+#pragma unroll
+  for (int k = 0; k < 64; k++) {
+    if (k < s) {
+      T p = rlane(r[k], k);
+      if (!(p > (T)0)) { bad = true; p = (T)1; }
+      const T dk = sqrt(p), rdk = (T)1 / dk;
+      const T lik = r[k] * rdk;
+      r[k] = lane == k ? dk : lik;
+      myrd = lane == k ? rdk : myrd;
+#pragma unroll
+      for (int j = k + 1; j < 64; j++) r[j] -= lik * rlane(r[k], j);
+    }
+  }
+  T* gw = gp;
+  asm volatile("" : "+s"(gw));
+  T* guw = gu;
+  asm volatile("" : "+s"(guw));
+  int ln = lane;
+  asm volatile("" : "+v"(ln));
+#pragma unroll
+  for (int c = 0; c < 64; c++) {
+    if (c < s) {
+      if (ln < m && ln >= c) gw[ln + c * m] = r[c];
+    } else if (c < m) {
+      if (ln >= c && ln < m) guw[(ln - s) + (c - s) * u] = r[c];
+    }
+  }
+  asm volatile("" : "+v"(ln));
+#pragma unroll
+  for (int q = 0; q < 64; q++) {
+    if (q < s) {
+      const T rls = rlane(myrd, q);
+      const T lis = ln > q ? r[q] : (T)0;
+      r[q] = -lis * rls;
+#pragma unroll
+      for (int c = 0; c < q; c++) r[c] -= lis * (rlane(r[c], q) * rls);
+    }
+  }
+  asm volatile("" : "+v"(ln));
+#pragma unroll
+  for (int c = 0; c < 64; c++) r[c] = c == ln ? myrd : r[c] * myrd;
+  T* iv = Inv + d.offI;
+  asm volatile("" : "+s"(iv));
+  asm volatile("" : "+v"(ln));
+#pragma unroll
+  for (int c = 0; c < 64; c++) iv[ln * 64 + c] = (ln < s && c < s && c <= ln) ? r[c] : (T)0;
+  if (lane == 0 && bad) atomicOr(info, 1);
+}
+
 // ---- MFMA tile GEMM ---------------------------------------------------------
 // 64x64 tile, 4 waves (2x2), each wave a 32x32 sub-tile of 2x2 16x16 MFMA
 // blocks.  K is staged through LDS in chunks of 16.  The MFMA is issued as
@@ -491,6 +571,7 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
   template __global__ void k_front_small<T>(const FrontDesc*, const int*, T*, T*, T*, int*); \
   template __global__ void k_potrf_diag<T>(const FrontDesc*, const PanelTask*, T*, T*, int*); \
   template __global__ void k_potrf_wave<T>(const FrontDesc*, const PanelTask*, T*, T*, int*); \
+  template __global__ void k_front_wave<T>(const FrontDesc*, const int*, T*, T*, T*, int*); \
   template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_trsm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*);
