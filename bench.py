@@ -235,6 +235,8 @@ def main():
     ap.add_argument("--no-varcoef", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--dump-ops", default=None, help="write the per-op profile of the FP64 "
+                    "factorization (kind, level, tasks, flops, bytes, ms) to this JSON file")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
@@ -312,6 +314,9 @@ def main():
             e["gbps"] = round(e["bytes"] / (ms * 1e-3) / 1e9, 1) if ms > 0 else None
         breakdown[name] = e
     log(f"breakdown: {breakdown}")
+    if args.dump_ops:
+        with open(args.dump_ops, "w") as fh:
+            json.dump({k: v.tolist() for k, v in prof.items()}, fh)
     gemm_avg_ms = tm.gemm_ms / max(tm.gemm_launches, 1)
     gemm_rate = tm.gemm_flops / (tm.gemm_ms * 1e-3) if tm.gemm_ms > 0 else 0.0
     log(f"profile: total {tm.total_ms:.2f} ms, front {tm.front_ms:.2f} ms, gemm {tm.gemm_ms:.2f} ms "

@@ -203,39 +203,100 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const FrontDesc* __restrict_
 
 // ---- one-wave Cholesky + inverse of a 64 x 64 diagonal block ----------------
 // Lane i holds row i of the block in 64 registers (loops fully unrolled, so
-// every register index and every source lane is a constant); values other
-// lanes need are broadcast with v_readlane.  No LDS, no barrier.  Updates
-// are applied to all lanes unmasked: entries above the diagonal take garbage
-// and are never read or stored (per-lane masks in the unrolled code would
-// cost a mask register per column).
+// every register index is a constant).  Column k is published through LDS
+// once per step and read back by every lane as broadcast loads (two values
+// per 16-byte load): a wave's LDS operations complete in issue order, so the
+// single wave needs no s_barrier, only the compiler fence below.  Updates
+// are applied unmasked: entries above the diagonal take garbage and are
+// never read or stored.
 //   Cholesky, right-looking (src/taucs_sn_llt.c:1112-1120 calls dpotrf on
-//   the same block): d = sqrt(a_kk), l_ik = a_ik / d, a_ij -= l_ik l_jk (lane
-//   j's l_jk is read after lane j has scaled it).
-//   Inverse X = L^{-1} in place, row sweep: before step s, lane s holds the
-//   unscaled row s of X (p_sc, c < s); lanes i > s take
-//   x_ic -= l_is p_sc / l_ss, and the rows are scaled by 1/l_ii at the end.
-// A kb < 64 block is padded with the identity, whose factor and inverse are
-// the identity.
+//   the same block): d = sqrt(a_kk), l_ik = a_ik / d, a_ij -= l_ik l_jk.
+//   Inverse X = L^{-1}: L goes to LDS row-major; lane c forms column c by
+//   forward substitution x_ic = (delta_ic - sum_{t<i} l_it x_tc) / l_ii, with
+//   the l_it as broadcast loads and x_tc in the registers L's row vacated.
+// A block narrower than 64 is padded with the identity, whose factor and
+// inverse are the identity.
+#define PW_LD 66
+
+__device__ __forceinline__ void wave_lds_fence()
+{
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <typename T, bool FRONT>
+__device__ __forceinline__ void wave_chol_inv(T (&r)[64], int lane, int nsteps, T* col, T* Ls,
+                                              T* rdiag, bool& bad)
+{
+#pragma clang loop unroll(full)
+  for (int k = 0; k < 64; k++) {
+    if (!FRONT || k < nsteps) {
+      T p = __shfl(r[k], k);
+      if (!(p > (T)0)) { bad = true; p = (T)1; }
+      const T dk = sqrt(p), rdk = (T)1 / dk;
+      const T lik = r[k] * rdk;
+      r[k] = lane == k ? dk : lik;
+      col[lane] = r[k];
+      rdiag[k] = rdk;
+      wave_lds_fence();
+      // the broadcasts go in batches of 16: all loads of a batch are issued
+      // before the first is used, so their latencies overlap
+#pragma clang loop unroll(full)
+      for (int j0 = k + 1; j0 < 64; j0 += 16) {
+        T v[16];
+#pragma clang loop unroll(full)
+        for (int q = 0; q < 16; q++)
+          if (j0 + q < 64) v[q] = col[j0 + q];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma clang loop unroll(full)
+        for (int q = 0; q < 16; q++)
+          if (j0 + q < 64) r[j0 + q] -= lik * v[q];
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
 template <typename T>
-__device__ __forceinline__ T rlane(T v, int l);
-template <>
-__device__ __forceinline__ double rlane<double>(double v, int l)
+__device__ __forceinline__ void wave_inverse(T (&r)[64], int lane, int n, T* Ls, const T* rdiag)
 {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-  return __hiloint2double(hi, lo);
+  // L rows to LDS; registers then hold column `lane` of the inverse
+#pragma clang loop unroll(full)
+  for (int c = 0; c < 64; c++) Ls[lane * PW_LD + c] = r[c];
+  wave_lds_fence();
+#pragma clang loop unroll(full)
+  for (int i = 0; i < 64; i++) {
+    if (i < n) {
+      const T* li = Ls + i * PW_LD;
+      T a0 = (i == lane) ? (T)1 : (T)0, a1 = (T)0;
+#pragma clang loop unroll(full)
+      for (int t0 = 0; t0 < i; t0 += 16) {
+        T v[16];
+#pragma clang loop unroll(full)
+        for (int q = 0; q < 16; q++)
+          if (t0 + q < i) v[q] = li[t0 + q];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma clang loop unroll(full)
+        for (int q = 0; q < 16; q += 2) {
+          if (t0 + q < i) a0 -= v[q] * r[t0 + q];
+          if (t0 + q + 1 < i) a1 -= v[q + 1] * r[t0 + q + 1];
+        }
+      }
+      r[i] = (a0 + a1) * rdiag[i];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
 }
-template <>
-__device__ __forceinline__ float rlane<float>(float v, int l)
-{
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
+
 template <typename T>
 __global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__ fr,
                                                    const PanelTask* __restrict__ tasks,
                                                    T* __restrict__ L, T* __restrict__ Inv,
                                                    int* __restrict__ info)
 {
+  __shared__ __align__(16) T col[64];
+  __shared__ __align__(16) T Ls[64 * PW_LD];
+  __shared__ __align__(16) T rdiag[64];
   const PanelTask t = tasks[blockIdx.x];
   const FrontDesc d = fr[t.front];
   const long long m = d.m;
@@ -246,53 +307,26 @@ __global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__
   for (int c = 0; c < 64; c++)
     r[c] = (lane < kb && c < kb) ? gp[lane + c * m] : (lane == c ? (T)1 : (T)0);
   bool bad = false;
-  T myrd = (T)1;   // 1 / l_ii of this lane's row
-#pragma unroll
-  for (int k = 0; k < 64; k++) {
-    T p = rlane(r[k], k);
-    if (!(p > (T)0)) { bad = true; p = (T)1; }
-    const T dk = sqrt(p), rdk = (T)1 / dk;
-    const T lik = r[k] * rdk;
-    r[k] = lane == k ? dk : lik;
-    myrd = lane == k ? rdk : myrd;
-#pragma unroll
-    for (int j = k + 1; j < 64; j++) r[j] -= lik * rlane(r[k], j);  // l_ik l_jk
-  }
+  wave_chol_inv<T, false>(r, lane, 64, col, Ls, rdiag, bad);
   T* gw = gp;  // a fresh copy, so the loads' addresses need not stay live
   asm volatile("" : "+s"(gw));
 #pragma unroll
   for (int c = 0; c < 64; c++)
     if (lane < kb && c < kb && lane >= c) gw[lane + c * m] = r[c];
-  // per-phase copies of the lane index keep the compiler from sharing (and
-  // keeping live) one phase's 64 lane comparisons in the next
-  int ln = lane;
-  asm volatile("" : "+v"(ln));
-#pragma unroll
-  for (int s = 0; s < 64; s++) {
-    const T rls = rlane(myrd, s);
-    const T lis = ln > s ? r[s] : (T)0;
-    r[s] = -lis * rls;
-#pragma unroll
-    for (int c = 0; c < s; c++) r[c] -= lis * (rlane(r[c], s) * rls);
-  }
-  asm volatile("" : "+v"(ln));
-#pragma unroll
-  for (int c = 0; c < 64; c++) r[c] = c == ln ? myrd : r[c] * myrd;
+  wave_inverse<T>(r, lane, 64, Ls, rdiag);
   T* iv = Inv + d.offI + (long long)(t.k0 / 64) * 4096;
   asm volatile("" : "+s"(iv));
-  asm volatile("" : "+v"(ln));
 #pragma unroll
-  for (int c = 0; c < 64; c++) iv[ln * 64 + c] = (ln < kb && c < kb && c <= ln) ? r[c] : (T)0;
+  for (int i = 0; i < 64; i++) iv[i * 64 + lane] = (lane < kb && i < kb) ? r[i] : (T)0;
   if (lane == 0 && bad) atomicOr(info, 1);
 }
 
-// Whole front with m <= 64 in one wavefront, the same register scheme as
-// k_potrf_wave: lane i holds row i of the m x m front [F1 .; F2 U] (panel
-// columns from L, update block from U, which already holds the children's
-// extend-adds).  The first s right-looking steps leave L in the panel columns
-// and the Schur complement U - F2 F2^T (the reference's dsyrk,
-// src/taucs_sn_llt.c:1178-1186) in the update block; then the s x s
-// diagonal block is inverted in place.  Rows and columns beyond m are
+// Whole front with m <= 64 in one wavefront: lane i holds row i of the
+// m x m front [F1 .; F2 U] (panel columns from L, the update block from U,
+// which already holds the children's extend-adds).  The first s steps leave
+// L in the panel columns and the Schur complement U - F2 F2^T (the
+// reference's dsyrk, src/taucs_sn_llt.c:1178-1186) in the update block; then
+// the s x s diagonal block is inverted.  Rows and columns beyond m are
 // identity padding.
 template <typename T>
 __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__ fr,
@@ -300,6 +334,9 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
                                                    T* __restrict__ L, T* __restrict__ U,
                                                    T* __restrict__ Inv, int* __restrict__ info)
 {
+  __shared__ __align__(16) T col[64];
+  __shared__ __align__(16) T Ls[64 * PW_LD];
+  __shared__ __align__(16) T rdiag[64];
   const FrontDesc d = fr[list[blockIdx.x]];
   const int s = d.s, m = d.m, u = m - s, lane = threadIdx.x;
   T* gp = L + d.offL;
@@ -316,20 +353,7 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
     r[c] = v;
   }
   bool bad = false;
-  T myrd = (T)1;
-#pragma unroll
-  for (int k = 0; k < 64; k++) {
-    if (k < s) {
-      T p = rlane(r[k], k);
-      if (!(p > (T)0)) { bad = true; p = (T)1; }
-      const T dk = sqrt(p), rdk = (T)1 / dk;
-      const T lik = r[k] * rdk;
-      r[k] = lane == k ? dk : lik;
-      myrd = lane == k ? rdk : myrd;
-#pragma unroll
-      for (int j = k + 1; j < 64; j++) r[j] -= lik * rlane(r[k], j);
-    }
-  }
+  wave_chol_inv<T, true>(r, lane, s, col, Ls, rdiag, bad);
   T* gw = gp;
   asm volatile("" : "+s"(gw));
   T* guw = gu;
@@ -344,25 +368,12 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
       if (ln >= c && ln < m) guw[(ln - s) + (c - s) * u] = r[c];
     }
   }
-  asm volatile("" : "+v"(ln));
-#pragma unroll
-  for (int q = 0; q < 64; q++) {
-    if (q < s) {
-      const T rls = rlane(myrd, q);
-      const T lis = ln > q ? r[q] : (T)0;
-      r[q] = -lis * rls;
-#pragma unroll
-      for (int c = 0; c < q; c++) r[c] -= lis * (rlane(r[c], q) * rls);
-    }
-  }
-  asm volatile("" : "+v"(ln));
-#pragma unroll
-  for (int c = 0; c < 64; c++) r[c] = c == ln ? myrd : r[c] * myrd;
+  wave_inverse<T>(r, ln, s, Ls, rdiag);
   T* iv = Inv + d.offI;
   asm volatile("" : "+s"(iv));
   asm volatile("" : "+v"(ln));
 #pragma unroll
-  for (int c = 0; c < 64; c++) iv[ln * 64 + c] = (ln < s && c < s && c <= ln) ? r[c] : (T)0;
+  for (int i = 0; i < 64; i++) iv[i * 64 + ln] = (ln < s && i < s) ? r[i] : (T)0;
   if (lane == 0 && bad) atomicOr(info, 1);
 }
 
