@@ -160,6 +160,8 @@ def bench_batch(T, count=512, lo=16, hi=128, steps=3, seed=7):
         t0 = time.perf_counter()
         Bt = T.Batch(mats, precision=prec)
         t_setup = time.perf_counter() - t0
+        tuned = Bt.autotune(candidates=(64, 128, 256), reps=2)
+        log(f"batch {name} autotune: {tuned}")
         n = Bt.n_total
         b = np.concatenate([T.random_uniform(m.n, 500 + q) for q, m in enumerate(mats)])
         if Mbd is None:
@@ -193,7 +195,8 @@ def bench_batch(T, count=512, lo=16, hi=128, steps=3, seed=7):
              "factor_gflops": round(Bt.fund_flops / float(np.mean(tf)) / 1e9, 2),
              "systems_per_s": round(count / (float(np.mean(tf)) + float(np.mean(ts))), 1),
              "max_system_backward_error": float(max(bw)),
-             "setup_s": round(t_setup, 2), "levels": Bt.n_levels}
+             "setup_s": round(t_setup, 2), "levels": Bt.n_levels,
+             "panel_width": tuned[0], "autotune_ms": {str(w): round(v, 3) for w, v in tuned[1].items()}}
         if prec != T.FP64:
             r["refine_iterations"] = int(info.iterations)
         res[name] = r
@@ -252,6 +255,9 @@ def main():
     PA = T.permute(A, perm, inv)
     F = T.factor_symbolic(PA)
     fi = F.info()
+    # the schedule autotuner picks the outer panel width (timed numeric runs)
+    tuned_w, tune_ms = F.autotune(PA, T.FP64, candidates=(64, 128, 256, 512), reps=2)
+    log(f"autotune: panel width {tuned_w}, candidates (ms) {tune_ms}")
     B = np.stack([T.random_uniform(n, 1 + r) for r in range(k)], 1)
     dB = T.DeviceArray(B)
     dX = T.DeviceArray(np.zeros((n, k)))
@@ -380,7 +386,9 @@ def main():
         "config": {"workload": f"mesh3d {side}x{side}x{side} 7-point Laplacian, genmmd ordering, "
                                f"numeric factor + {k}-rhs supernodal solve (plain FP64 path)",
                    "n": n, "nrhs": k, "fund_flops": fi.fund_flops, "relax_flops": fi.relax_flops,
-                   "n_supernodes": fi.n_live, "levels": fi.n_levels, "max_front": fi.max_front},
+                   "n_supernodes": fi.n_live, "levels": fi.n_levels, "max_front": fi.max_front,
+                   "panel_width": tuned_w,
+                   "autotune_ms": {str(w): round(v, 3) for w, v in tune_ms.items()}},
         "factor_ms": round(1e3 * tf_mean, 3),
         "solve_ms": round(1e3 * ts_mean, 3),
         "solves_per_s": round(k / ts_mean, 2),

@@ -111,6 +111,19 @@ int   taucs_amd_batch_info(void* B, int* count, int* n_total, long long* xoff,
 void* taucs_amd_batch_system_factor(void* B, int q);
 void  taucs_amd_batch_free(void* B);
 
+/* ---- schedule shape and its autotuner ------------------------------------ */
+/* Large fronts are factored in outer panels of w columns (64-column POTRF /
+   TRSM steps inside, one trailing update of depth w per panel).  Default 256;
+   applies to schedules built afterwards. */
+void  taucs_amd_set_panel_width(int w);
+/* Time the numeric factorization of A with each candidate panel width
+   (multiples of 64) and keep the fastest schedule in L, factored.  ms[i]:
+   best device time of candidate i over `reps` runs; *best_w: the choice. */
+int   taucs_amd_autotune(void* L, taucs_ccs_matrix* A, int precision, const int* cand,
+                         int ncand, int reps, double* ms, int* best_w);
+int   taucs_amd_batch_autotune(void* B, const int* cand, int ncand, int reps, double* ms,
+                               int* best_w);
+
 /* hipGraph capture of the numeric schedule and of the solves (default on) */
 void  taucs_amd_set_graphs(int on);
 /* Panel lookahead: the trailing update of a panel step runs on a second stream

@@ -98,6 +98,11 @@ _sigs = {
     "taucs_amd_set_graphs": (None, [C.c_int]),
     "taucs_amd_set_lookahead": (None, [C.c_int]),
     "taucs_amd_set_potrf_kernel": (None, [C.c_int]),
+    "taucs_amd_set_panel_width": (None, [C.c_int]),
+    "taucs_amd_autotune": (C.c_int, [C.c_void_p, _ccs_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                     C.c_void_p, C.c_void_p]),
+    "taucs_amd_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                           C.c_void_p]),
     "taucs_amd_get_op_profile": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 6),
     "taucs_amd_batch_factor_llt_mf": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int]),
     "taucs_amd_batch_numeric": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
@@ -341,6 +346,16 @@ class Factor:
         lib.taucs_amd_get_numeric_timing(self.h, C.byref(t))
         return t
 
+    def autotune(self, A, precision=FP64, candidates=(64, 128, 256, 512), reps=2):
+        """Pick the outer panel width by timing the numeric phase; L ends factored."""
+        cand = np.asarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(cand))
+        best = C.c_int()
+        if lib.taucs_amd_autotune(self.h, A.ptr, precision, _dptr(cand), len(cand), reps, _dptr(ms),
+                                  C.byref(best)):
+            raise RuntimeError("autotune failed")
+        return best.value, dict(zip(cand.tolist(), ms.tolist()))
+
     def op_profile(self):
         """Per-op kind, level, tasks, flops, bytes and ms of the last profiled numeric run."""
         n = lib.taucs_amd_get_op_profile(self.h, 0, None, None, None, None, None, None)
@@ -489,6 +504,16 @@ class Batch:
                                              B.shape[0], method, tol, maxit, C.byref(info)):
             raise RuntimeError("batched refined solve failed")
         return (X[:, 0].copy() if one else X), info
+
+    def autotune(self, candidates=(64, 128, 256, 512), reps=2):
+        """Pick the batch schedule's outer panel width by timing; ends factored."""
+        cand = np.asarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(cand))
+        best = C.c_int()
+        if lib.taucs_amd_batch_autotune(self.h, _dptr(cand), len(cand), reps, _dptr(ms),
+                                        C.byref(best)):
+            raise RuntimeError("batch autotune failed")
+        return best.value, dict(zip(cand.tolist(), ms.tolist()))
 
     def system_supernodes(self, q):
         f = Factor(lib.taucs_amd_batch_system_factor(self.h, q))

@@ -59,6 +59,7 @@ struct Op {
   int start, count;  // range in the task array of that kind
   int level;
   int lane = 0;      // 1: may run on the side stream (the lookahead's trailing update)
+  bool nojoin = false;  // touches only the current outer panel: need not wait for the side stream
 };
 
 // solve op kinds
@@ -89,6 +90,7 @@ struct tamd_dev {
   std::vector<float> op_ms;                // device time of each op (profiling runs)
   std::vector<int> lvl_ptr;      // fronts of level l: [lvl_ptr[l], lvl_ptr[l+1])
   int n_fronts = 0, ncol = 0, nnz = 0, n_levels = 0;
+  int panel_w = 256;              // outer panel width of large fronts (multiple of 64)
   long long W_per_rhs = 0;
   size_t struct_bytes = 0;       // device bytes of the schedule and A values
   double gemm_flops = 0, assembly_bytes = 0;
@@ -130,6 +132,12 @@ struct tamd_dev {
 
 static int g_graphs = 1;
 extern "C" void taucs_amd_set_graphs(int on) { g_graphs = on; }
+static int g_panel_w = 256;
+extern "C" void taucs_amd_set_panel_width(int w)
+{
+  w = (w + 63) / 64 * 64;
+  g_panel_w = w < 64 ? 64 : (w > 1024 ? 1024 : w);
+}
 static int g_lookahead = 1;
 extern "C" void taucs_amd_set_lookahead(int on) { g_lookahead = on; }
 // Diagonal-block kernel: 1 = one wave with register broadcasts, 0 = 256
@@ -277,8 +285,9 @@ struct SysView {
 // Builds the device schedule of one or more independent systems.  Fronts of
 // level l of every system share the launches of level l; each system keeps
 // its own arenas (offsets), row numbering (shifted by xoff) and tree.
-static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = false)
+static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = false, int panel_w = 0)
 {
+  D->panel_w = panel_w > 0 ? panel_w : g_panel_w;
   int nlev = 0, nf = 0;
   long long nnz_tot = 0;
   for (const SysView& v : sys) {
@@ -394,57 +403,91 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       zero.insert(zero.end(), zero_at[l].begin(), zero_at[l].end());
     }
     int st = (int)small.size();
-    int maxsteps = 0;
+    int maxouter = 0;
+    const int W = D->panel_w;
     for (int k = b0; k < b1; k++) {
       if (fr[k].m <= SMALL_M) small.push_back(k);
-      else maxsteps = std::max(maxsteps, (fr[k].s + NB - 1) / NB);
+      else maxouter = std::max(maxouter, (fr[k].s + W - 1) / W);
     }
     if ((int)small.size() > st) D->ops.push_back({TAMD_OP_SMALL, st, (int)small.size() - st, l});
-    // Lookahead: the trailing update of a step is split into the tiles of the
-    // next panel (main stream) and the rest (side stream), so the next
-    // panel's POTRF and TRSM overlap the rest of this step's update.
+    // Two-level blocking: outer panels of W columns, each factored in 64-column
+    // steps (POTRF of the diagonal block, TRSM of the rows below by its
+    // inverse, update of the panel's remaining columns with K = 64); then one
+    // trailing update of the columns right of the panel with K = W, so the
+    // trailing matrix is read and written once per W columns.
+    // Lookahead: that update is split into the next outer panel's columns
+    // (main stream) and the rest (side stream); the next panel's own steps
+    // touch only its columns and overlap the rest.
+    auto tile = [&](const FrontDesc& d, int r0, int c0, int k0, int kw) {
+      GemmTask g;
+      g.a_off = d.offL + r0 + (long long)k0 * d.m;
+      g.b_off = d.offL + c0 + (long long)k0 * d.m;
+      g.c_off = d.offL + r0 + (long long)c0 * d.m;
+      g.ld = d.m;
+      g.ldc = d.m;
+      g.M = std::min(64, d.m - r0);
+      g.N = std::min(64, d.s - c0);
+      g.K = kw;
+      g.flags = (r0 == c0) ? 2 : 0;
+      D->gemm_flops += tile_flops(g);
+      return g;
+    };
     std::vector<GemmTask> rest;
-    for (int step = 0; step < maxsteps; step++) {
-      int p0 = (int)potrf.size(), t0 = (int)trsm.size(), g0 = (int)gemm.size();
+    for (int os = 0; os < maxouter; os++) {
+      for (int j = 0; j < W / NB; j++) {
+        int p0 = (int)potrf.size(), t0 = (int)trsm.size(), g0 = (int)gemm.size();
+        for (int k = b0; k < b1; k++) {
+          const FrontDesc& d = fr[k];
+          if (d.m <= SMALL_M) continue;
+          const int K0 = os * W, k0 = K0 + j * NB;
+          if (k0 >= d.s) continue;
+          const int kb = std::min(NB, d.s - k0), pend = std::min(K0 + W, d.s);
+          potrf.push_back({k, k0, kb, 0, 0});
+          for (int r0 = k0 + kb; r0 < d.m; r0 += 64) {
+            GemmTask g;
+            g.a_off = d.offL + r0 + (long long)k0 * d.m;
+            g.b_off = d.offI + (long long)(k0 / 64) * 4096;
+            g.c_off = g.a_off;
+            g.ld = d.m;
+            g.ldc = d.m;
+            g.M = std::min(64, d.m - r0);
+            g.N = kb;
+            g.K = kb;
+            g.flags = 4;
+            trsm.push_back(g);
+          }
+          for (int c0 = k0 + kb; c0 < pend; c0 += 64)
+            for (int r0 = c0; r0 < d.m; r0 += 64) gemm.push_back(tile(d, r0, c0, k0, kb));
+        }
+        Op o;
+        if ((int)potrf.size() > p0) {
+          o = {TAMD_OP_POTRF, p0, (int)potrf.size() - p0, l};
+          o.nojoin = true;
+          D->ops.push_back(o);
+        }
+        if ((int)trsm.size() > t0) {
+          o = {TAMD_OP_TRSM, t0, (int)trsm.size() - t0, l};
+          o.nojoin = true;
+          D->ops.push_back(o);
+        }
+        if ((int)gemm.size() > g0) {
+          o = {TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l};
+          o.nojoin = true;
+          D->ops.push_back(o);
+        }
+      }
+      int g0 = (int)gemm.size();
       rest.clear();
       for (int k = b0; k < b1; k++) {
         const FrontDesc& d = fr[k];
         if (d.m <= SMALL_M) continue;
-        int k0 = step * NB;
-        if (k0 >= d.s) continue;
-        int kb = std::min(NB, d.s - k0);
-        potrf.push_back({k, k0, kb, 0, 0});
-        for (int r0 = k0 + kb; r0 < d.m; r0 += 64) {
-          GemmTask g;
-          g.a_off = d.offL + r0 + (long long)k0 * d.m;
-          g.b_off = d.offI + (long long)(k0 / 64) * 4096;
-          g.c_off = g.a_off;
-          g.ld = d.m;
-          g.ldc = d.m;
-          g.M = std::min(64, d.m - r0);
-          g.N = kb;
-          g.K = kb;
-          g.flags = 4;
-          trsm.push_back(g);
-        }
-        for (int c0 = k0 + kb; c0 < d.s; c0 += 64)
-          for (int r0 = c0; r0 < d.m; r0 += 64) {
-            GemmTask g;
-            g.a_off = d.offL + r0 + (long long)k0 * d.m;
-            g.b_off = d.offL + c0 + (long long)k0 * d.m;
-            g.c_off = d.offL + r0 + (long long)c0 * d.m;
-            g.ld = d.m;
-            g.ldc = d.m;
-            g.M = std::min(64, d.m - r0);
-            g.N = std::min(64, d.s - c0);
-            g.K = kb;
-            g.flags = (r0 == c0) ? 2 : 0;
-            (c0 == k0 + kb ? gemm : rest).push_back(g);
-            D->gemm_flops += tile_flops(g);
-          }
+        const int K0 = os * W;
+        if (K0 >= d.s) continue;
+        const int kw = std::min(W, d.s - K0);
+        for (int c0 = K0 + W; c0 < d.s; c0 += 64)
+          for (int r0 = c0; r0 < d.m; r0 += 64)
+            (c0 < K0 + 2 * W ? gemm : rest).push_back(tile(d, r0, c0, K0, kw));
       }
-      if ((int)potrf.size() > p0) D->ops.push_back({TAMD_OP_POTRF, p0, (int)potrf.size() - p0, l});
-      if ((int)trsm.size() > t0) D->ops.push_back({TAMD_OP_TRSM, t0, (int)trsm.size() - t0, l});
       if ((int)gemm.size() > g0) D->ops.push_back({TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l});
       if (!rest.empty()) {
         Op o = {TAMD_OP_GEMM, (int)gemm.size(), (int)rest.size(), l};
@@ -694,7 +737,7 @@ extern "C" int tamd_dev_batch_setup(tamd_factor* FB, int count, tamd_factor* con
     v[q].P = &F[q]->plan;
     v[q].xoff = xoff[q];
   }
-  if (dev_build(D, v) != 0) {
+  if (dev_build(D, v, false, FB->panel_w) != 0) {
     tamd_dev_free(FB);
     return -1;
   }
@@ -717,7 +760,7 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
   one[0].S = &F->sym;
   one[0].P = &F->plan;
   one[0].xoff = 0;
-  if (dev_build(D, one) != 0) {
+  if (dev_build(D, one, false, F->panel_w) != 0) {
     tamd_dev_free(F);
     return -1;
   }
@@ -737,7 +780,7 @@ extern "C" long long taucs_amd_factor_device_bytes(void* vL, int precision, int 
   one[0].S = &F->sym;
   one[0].P = &F->plan;
   one[0].xoff = 0;
-  if (dev_build(&D, one, true) != 0) return -1;
+  if (dev_build(&D, one, true, F->panel_w) != 0) return -1;
   const long long elem = precision == TAMD_PREC_FP64 ? 8 : 4;
   const long long n = F->n, cap = std::max(nrhs, 16);
   long long b = (long long)D.struct_bytes + sizeof(int);
@@ -866,7 +909,7 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
   const bool la = !prof && g_lookahead && D->side;
   for (const Op& op : D->ops) {
     const int lane = la ? op.lane : 0;
-    const bool join = !(op.kind == TAMD_OP_POTRF || op.kind == TAMD_OP_TRSM);
+    const bool join = !op.nojoin;
     if (prof) HIPCHECK(hipEventRecord(a0, E.main_st));
     switch (op.kind) {
       case TAMD_OP_ZERO:
@@ -1273,6 +1316,33 @@ extern "C" int tamd_dev_download_blocks(tamd_factor* F, double* Lhost)
     HIPCHECK(hipStreamSynchronize(D->stream));
     for (size_t i = 0; i < n; i++) Lhost[i] = tmp[i];
   }
+  return 0;
+}
+
+// ---- timing for the schedule autotuner ---------------------------------------
+// reps + 1 numeric factorizations from host values; the first (which builds
+// the graph) is not counted; *best_ms = the fastest of the others, by device
+// events around each replay on the factor's stream.
+extern "C" int tamd_dev_time_numeric(tamd_factor* F, const double* vals, int reps, double* best_ms)
+{
+  double t = 1e300;
+  if (!F->dev) return -1;
+  for (int r = 0; r < reps + 1; r++) {
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, F->dev->stream));
+    const int rc = tamd_dev_numeric_values(F, vals);
+    HIPCHECK(hipEventRecord(e1, F->dev->stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != 0) return -1;
+    if (r > 0 || reps == 0) t = std::min(t, (double)ms);
+  }
+  *best_ms = t;
   return 0;
 }
 

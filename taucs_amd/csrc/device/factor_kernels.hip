@@ -218,6 +218,30 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const FrontDesc* __restrict_
 // inverse are the identity.
 #define PW_LD 66
 
+// v_readlane broadcast of one lane's value (lane index a constant)
+template <typename T>
+__device__ __forceinline__ T rlane(T v, int l);
+template <>
+__device__ __forceinline__ double rlane<double>(double v, int l)
+{
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+template <>
+__device__ __forceinline__ float rlane<float>(float v, int l)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// d = sqrt(p) and 1/d, both correctly rounded (as the reference's dpotrf)
+template <typename T>
+__device__ __forceinline__ void sqrt_pair(T p, T& d, T& rd)
+{
+  d = sqrt(p);
+  rd = (T)1 / d;
+}
+
 __device__ __forceinline__ void wave_lds_fence()
 {
   __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
@@ -231,9 +255,10 @@ __device__ __forceinline__ void wave_chol_inv(T (&r)[64], int lane, int nsteps, 
 #pragma clang loop unroll(full)
   for (int k = 0; k < 64; k++) {
     if (!FRONT || k < nsteps) {
-      T p = __shfl(r[k], k);
+      T p = rlane(r[k], k);
       if (!(p > (T)0)) { bad = true; p = (T)1; }
-      const T dk = sqrt(p), rdk = (T)1 / dk;
+      T dk, rdk;
+      sqrt_pair(p, dk, rdk);
       const T lik = r[k] * rdk;
       r[k] = lane == k ? dk : lik;
       col[lane] = r[k];

@@ -55,6 +55,9 @@ __global__ void k_csr_absrow(int n, const int* rp, const double* v, double* out)
   out[i] = acc;
 }
 
+// max that propagates NaN (fmax would drop it and hide a diverged iterate)
+__device__ __forceinline__ double maxnan(double x, double y) { return (x != x || x > y) ? x : y; }
+
 // Per-column partial reductions: mode 0 sum(a*b), 1 max|a|, 2 sum(a*a).
 // partial[(blk) + r*gridDim.x]
 __global__ __launch_bounds__(256) void k_colreduce(int n, const double* __restrict__ a, int lda,
@@ -67,14 +70,14 @@ __global__ __launch_bounds__(256) void k_colreduce(int n, const double* __restri
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     double av = a[i + (long long)r * lda];
     if (mode == 0) acc += av * b[i + (long long)r * ldb];
-    else if (mode == 1) acc = fmax(acc, fabs(av));
+    else if (mode == 1) acc = maxnan(acc, fabs(av));
     else acc += av * av;
   }
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
     if (threadIdx.x < off)
-      red[threadIdx.x] = mode == 1 ? fmax(red[threadIdx.x], red[threadIdx.x + off])
+      red[threadIdx.x] = mode == 1 ? maxnan(red[threadIdx.x], red[threadIdx.x + off])
                                    : red[threadIdx.x] + red[threadIdx.x + off];
     __syncthreads();
   }
@@ -87,7 +90,7 @@ __global__ void k_colfinal(int nblk, int mode, const double* partial, double* ou
     double acc = 0.0;
     for (int q = 0; q < nblk; q++) {
       double v = partial[q + (long long)r * nblk];
-      acc = mode == 1 ? fmax(acc, v) : acc + v;
+      acc = mode == 1 ? maxnan(acc, v) : acc + v;
     }
     out[r] = acc;
   }
@@ -374,13 +377,16 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
     if (tamd_dev_solve_device(F, nrhs, c->x, n, c->b, n)) return -1;
     for (;;) {
       if (residual_and_error(st, c, c->x, c->b, c->r, nrhs, eta, relres)) return -1;
-      bool all = true;
+      bool all = true, settled = true;
       for (int q = 0; q < nrhs; q++) {
         if (eta[q] <= tol) done[q] = 1;
-        mask[q] = done[q] ? 0.0 : 1.0;
+        const bool diverged = !(eta[q] <= 1.0);  // NaN or inf (eta <= 1 otherwise)
+        mask[q] = (done[q] || diverged) ? 0.0 : 1.0;
         all = all && done[q];
+        settled = settled && (done[q] || diverged);
       }
       if (all) { conv = true; break; }
+      if (settled) break;
       if (it >= maxit) break;
       if (tamd_dev_solve_device(F, nrhs, c->z, n, c->r, n)) return -1;
       HIPOK(hipMemcpyAsync(c->mask, mask.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, st));

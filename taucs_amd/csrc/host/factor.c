@@ -353,3 +353,35 @@ int taucs_amd_factor_get_schedule(void* vL, int* sn_level, int* sn_parent, long 
   }
   return 0;
 }
+
+/* Autotuner of the schedule shape: the numeric phase is timed with each
+   candidate outer panel width (the trailing-update depth of the large
+   fronts), and the fastest schedule is kept in L, factored from A.
+   cand: candidate widths (multiples of 64); ms[i] receives candidate i's
+   best time over `reps` timed runs. */
+int taucs_amd_autotune(void* vL, taucs_ccs_matrix* A, int precision, const int* cand, int ncand,
+                       int reps, double* ms, int* best_w)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  int c, best = -1;
+  double best_ms = 1e300;
+  if (!F || !A || !cand || ncand <= 0 || check_input(A, "taucs_amd_autotune") != 0) return -1;
+  if (A->n != F->n || tamd_pattern_hash(A) != F->plan.a_pattern_hash) return -1;
+  F->precision = precision;
+  for (c = 0; c < ncand; c++) {
+    double t;
+    if (cand[c] < 64 || cand[c] % 64) return -1;
+    tamd_dev_free(F);
+    F->has_numeric = 0;
+    F->panel_w = cand[c];
+    if (tamd_dev_setup(F) != 0 || tamd_dev_time_numeric(F, A->values.d, reps, &t) != 0) return -1;
+    if (ms) ms[c] = t;
+    if (t < best_ms) { best_ms = t; best = c; }
+  }
+  tamd_dev_free(F);
+  F->panel_w = cand[best];
+  if (tamd_dev_setup(F) != 0 || tamd_dev_numeric(F, A) != 0) return -1;
+  F->has_numeric = 1;
+  if (best_w) *best_w = cand[best];
+  return 0;
+}

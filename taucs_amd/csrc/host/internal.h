@@ -86,6 +86,7 @@ typedef struct tamd_factor {
   struct tamd_dev* dev;      /* device-resident numeric state */
   int            has_numeric;
   double         t_symbolic, t_numeric; /* seconds */
+  int            panel_w;    /* outer panel width of the schedule (0: library default) */
 } tamd_factor;
 
 /* ---- host symbolic layer (symbolic.c, mmd.c) ---- */
@@ -97,6 +98,7 @@ int  tamd_plan_build(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_pla
 int  tamd_plan_amap(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_plan* P);
 void tamd_plan_free(tamd_plan* P);
 uint64_t tamd_pattern_hash(const taucs_ccs_matrix* A);
+int  tamd_dev_time_numeric(struct tamd_factor* F, const double* vals, int reps, double* best_ms);
 
 /* ---- device numeric layer (the HIP sources under device/) ---- */
 int  tamd_dev_available(void);

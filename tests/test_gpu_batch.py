@@ -147,3 +147,27 @@ def test_refinement_follows_refactorization():
     x3, i3 = F.solve_refined(PA, b, method=1, tol=1e-14, maxit=100)
     assert i3.converged
     assert np.allclose(x3, x1, rtol=1e-9, atol=1e-12)
+
+
+def test_autotune_picks_a_candidate_and_keeps_a_correct_factor():
+    """The schedule autotuner times each outer panel width and keeps the
+    fastest; every width gives the same factorization up to rounding."""
+    print("case: schedule autotuner", flush=True)
+    A = T.mesh3d(22)
+    perm, inv = T.order(A, "genmmd")
+    PA = T.permute(A, perm, inv)
+    b = T.random_uniform(A.n, 3)
+    x0 = T.factor_llt_mf(PA).solve(b)
+    F = T.factor_symbolic(PA)
+    best, ms = F.autotune(PA, candidates=(64, 128, 256), reps=1)
+    assert best in (64, 128, 256) and all(v > 0 for v in ms.values())
+    assert ms[best] == min(ms.values())
+    x = F.solve(b)
+    assert np.linalg.norm(x - x0) / np.linalg.norm(x0) <= 1e-10
+    mats = shifted_systems(12, 16, 64, 5)
+    B = T.Batch(mats)
+    bb = np.concatenate([T.random_uniform(m.n, q) for q, m in enumerate(mats)])
+    xb0 = B.solve(bb)
+    best, ms = B.autotune(candidates=(64, 192), reps=1)
+    assert best in (64, 192)
+    assert np.allclose(B.solve(bb), xb0, rtol=1e-10, atol=1e-13)
