@@ -77,6 +77,9 @@ int taucs_amd_supernodal_solve_many_device(void* L, int nrhs, double* dX, int ld
 #define TAUCS_AMD_BF16UPD 2
 void* taucs_amd_factor_llt_mf_precision(taucs_ccs_matrix* A, int precision);
 int   taucs_amd_factor_llt_numeric_precision(taucs_ccs_matrix* A, void* L, int precision);
+/* Numeric factorization from A's values already in device memory (nnz
+   doubles in the stored order of the permuted matrix of the symbolic factor). */
+int   taucs_amd_factor_llt_numeric_device(void* L, const double* d_vals, int precision);
 
 typedef struct {
   int    iterations;     /* outer iterations used (max over right-hand sides) */
@@ -99,6 +102,7 @@ int taucs_amd_solve_refined(void* L, taucs_ccs_matrix* A, int nrhs, double* X, i
    Vectors are concatenated: system q owns rows xoff[q] .. xoff[q+1]-1. */
 void* taucs_amd_batch_factor_llt_mf(int count, taucs_ccs_matrix** A, int precision);
 int   taucs_amd_batch_numeric(void* B, taucs_ccs_matrix** A, int precision);
+int   taucs_amd_batch_numeric_device(void* B, const double* d_vals, int precision);
 int   taucs_amd_batch_solve(void* B, int nrhs, double* X, int ldx, const double* Bv, int ldb);
 int   taucs_amd_batch_solve_device(void* B, int nrhs, double* dX, int ldx, const double* dB,
                                    int ldb);
@@ -138,6 +142,7 @@ void* taucs_amd_device_malloc(size_t bytes);
 void  taucs_amd_device_free(void* p);
 int   taucs_amd_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int   taucs_amd_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int   taucs_amd_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int   taucs_amd_device_synchronize(void);
 
 /* ---- device memory accounting (admission by HBM budget) ----------------- */

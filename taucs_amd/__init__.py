@@ -99,6 +99,9 @@ _sigs = {
     "taucs_amd_set_lookahead": (None, [C.c_int]),
     "taucs_amd_set_potrf_kernel": (None, [C.c_int]),
     "taucs_amd_set_panel_width": (None, [C.c_int]),
+    "taucs_amd_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "taucs_amd_factor_llt_numeric_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "taucs_amd_batch_numeric_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "taucs_amd_autotune": (C.c_int, [C.c_void_p, _ccs_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
                                      C.c_void_p, C.c_void_p]),
     "taucs_amd_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
@@ -346,6 +349,10 @@ class Factor:
         lib.taucs_amd_get_numeric_timing(self.h, C.byref(t))
         return t
 
+    def numeric_device(self, d_vals, precision=FP64):
+        """Numeric factorization from values already in device memory."""
+        return lib.taucs_amd_factor_llt_numeric_device(self.h, d_vals, precision)
+
     def autotune(self, A, precision=FP64, candidates=(64, 128, 256, 512), reps=2):
         """Pick the outer panel width by timing the numeric phase; L ends factored."""
         cand = np.asarray(candidates, dtype=np.int32)
@@ -504,6 +511,9 @@ class Batch:
                                              B.shape[0], method, tol, maxit, C.byref(info)):
             raise RuntimeError("batched refined solve failed")
         return (X[:, 0].copy() if one else X), info
+
+    def numeric_device(self, d_vals, precision=FP64):
+        return lib.taucs_amd_batch_numeric_device(self.h, d_vals, precision)
 
     def autotune(self, candidates=(64, 128, 256, 512), reps=2):
         """Pick the batch schedule's outer panel width by timing; ends factored."""

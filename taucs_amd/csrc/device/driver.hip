@@ -1097,6 +1097,20 @@ extern "C" int tamd_dev_values_of(tamd_factor* F, const double** d_avals, const 
   return 0;
 }
 
+// Numeric factorization from values already in device memory (same order as
+// the permuted matrix's stored entries; for a batch, the concatenation).
+extern "C" int tamd_dev_numeric_device(tamd_factor* F, const double* d_vals)
+{
+  tamd_dev* D = F->dev;
+  if (!D || ensure_numeric_buffers(F) != 0) return -1;
+  HIPCHECK(hipMemcpyAsync(D->d_avals, d_vals, sizeof(double) * (size_t)D->nnz, hipMemcpyDeviceToDevice,
+                          D->stream));
+  D->num_vals = nullptr;  // no host array holds these values
+  D->num_gen++;
+  if (F->precision == TAMD_PREC_FP64) return run_numeric<double>(D, false);
+  return run_numeric<float>(D, F->precision == TAMD_PREC_BF16UPD);
+}
+
 extern "C" int tamd_dev_numeric(tamd_factor* F, const taucs_ccs_matrix* A)
 {
   return tamd_dev_numeric_values(F, A->values.d);
@@ -1357,6 +1371,10 @@ extern "C" void taucs_amd_device_free(void* p) { free_ptr(p); }
 extern "C" int taucs_amd_memcpy_h2d(void* dst, const void* src, size_t bytes)
 {
   return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+}
+extern "C" int taucs_amd_memcpy_d2d(void* dst, const void* src, size_t bytes)
+{
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? 0 : -1;
 }
 extern "C" int taucs_amd_memcpy_d2h(void* dst, const void* src, size_t bytes)
 {

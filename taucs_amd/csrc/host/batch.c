@@ -21,6 +21,7 @@
 
 int tamd_dev_batch_setup(tamd_factor* FB, int count, tamd_factor* const* F, const long long* xoff);
 int tamd_dev_numeric_values(tamd_factor* F, const double* vals);
+int tamd_dev_numeric_device(tamd_factor* F, const double* d_vals);
 void* taucs_dccs_factor_llt_symbolic(taucs_ccs_matrix* A);
 
 typedef struct {
@@ -220,5 +221,22 @@ int taucs_amd_batch_autotune(void* vB, const int* cand, int ncand, int reps, dou
     return -1;
   B->all.has_numeric = 1;
   if (best_w) *best_w = cand[best];
+  return 0;
+}
+
+/* Numeric phase of a batch from the concatenated values already in device
+   memory (system q's stored entries at the offset of its nnz prefix sum). */
+int taucs_amd_batch_numeric_device(void* vB, const double* d_vals, int precision)
+{
+  tamd_batch* B = (tamd_batch*)vB;
+  double t0 = taucs_wtime();
+  if (!B || !d_vals) return -1;
+  B->all.precision = precision;
+  if (tamd_dev_numeric_device(&B->all, d_vals) != 0) {
+    B->all.has_numeric = 0;
+    return -1;
+  }
+  B->all.has_numeric = 1;
+  B->all.t_numeric = taucs_wtime() - t0;
   return 0;
 }

@@ -385,3 +385,23 @@ int taucs_amd_autotune(void* vL, taucs_ccs_matrix* A, int precision, const int* 
   if (best_w) *best_w = cand[best];
   return 0;
 }
+
+/* Numeric factorization from A's values already in device memory (d_vals:
+   nnz doubles in the stored order of the permuted matrix the symbolic
+   factor was built from), for callers whose inputs live in HBM. */
+int taucs_amd_factor_llt_numeric_device(void* vL, const double* d_vals, int precision)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  double t0 = taucs_wtime();
+  if (!F || !d_vals) return -1;
+  F->precision = precision;
+  if (!F->dev && tamd_dev_setup(F) != 0) return -1;
+  if (tamd_dev_numeric_device(F, d_vals) != 0) {
+    tamd_dev_free_numeric(F);
+    F->has_numeric = 0;
+    return -1;
+  }
+  F->has_numeric = 1;
+  F->t_numeric = taucs_wtime() - t0;
+  return 0;
+}

@@ -104,6 +104,7 @@ int  tamd_dev_time_numeric(struct tamd_factor* F, const double* vals, int reps, 
 int  tamd_dev_available(void);
 int  tamd_dev_setup(tamd_factor* F);
 int  tamd_dev_numeric(tamd_factor* F, const taucs_ccs_matrix* A);
+int  tamd_dev_numeric_device(tamd_factor* F, const double* d_vals);
 int  tamd_dev_solve(tamd_factor* F, int nrhs, double* X, int ldx, const double* B, int ldb);
 int  tamd_dev_download_blocks(tamd_factor* F, double* Lhost);
 void tamd_dev_free_numeric(tamd_factor* F);

@@ -171,3 +171,24 @@ def test_autotune_picks_a_candidate_and_keeps_a_correct_factor():
     best, ms = B.autotune(candidates=(64, 192), reps=1)
     assert best in (64, 192)
     assert np.allclose(B.solve(bb), xb0, rtol=1e-10, atol=1e-13)
+
+
+def test_numeric_from_device_values_equals_host_values():
+    print("case: numeric from device-resident values", flush=True)
+    A = T.mesh3d(16)
+    perm, inv = T.order(A, "genmmd")
+    PA = T.permute(A, perm, inv)
+    b = T.random_uniform(A.n, 8)
+    F1 = T.factor_llt_mf(PA)
+    x1 = F1.solve(b)
+    F2 = T.factor_symbolic(PA)
+    dV = T.DeviceArray(PA.arrays()[3])
+    assert F2.numeric_device(dV.ptr) == 0
+    assert np.array_equal(F2.solve(b), x1)
+    mats = shifted_systems(6, 16, 40, 9)
+    B1 = T.Batch(mats)
+    bb = np.concatenate([T.random_uniform(m.n, q) for q, m in enumerate(mats)])
+    xb = B1.solve(bb)
+    dVb = T.DeviceArray(np.concatenate([m.arrays()[3] for m in mats]))
+    assert B1.numeric_device(dVb.ptr) == 0
+    assert np.array_equal(B1.solve(bb), xb)
