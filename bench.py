@@ -350,6 +350,13 @@ def main():
         with open(args.dump_ops, "w") as fh:
             json.dump({k: v.tolist() for k, v in prof.items()}, fh)
     gemm_avg_ms = tm.gemm_ms / max(tm.gemm_launches, 1)
+    # memory traffic per launch of the tile GEMM, from the committed counter
+    # passes (FETCH_SIZE / WRITE_SIZE, calibrated by a 1 GiB copy) of this workload
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "c2_gemm_traffic.json")
+    if side == 60 and os.path.exists(tp):
+        with open(tp) as fh:
+            traffic = round(json.load(fh)["bytes_per_launch"] / 1e6, 2)
     gemm_rate = tm.gemm_flops / (tm.gemm_ms * 1e-3) if tm.gemm_ms > 0 else 0.0
     log(f"profile: total {tm.total_ms:.2f} ms, front {tm.front_ms:.2f} ms, gemm {tm.gemm_ms:.2f} ms "
         f"({tm.gemm_launches} launches, {gemm_rate/1e12:.2f} TFLOP/s), assembly {tm.assembly_ms:.2f} ms, "
@@ -428,7 +435,8 @@ def main():
                      "achieved": round(gemm_rate / 1e12, 3), "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": round(gemm_rate / PEAK_FP64, 4),
                      "avg_launch_ms": round(gemm_avg_ms, 4), "launches": tm.gemm_launches,
-                     "traffic": None},
+                     "traffic": traffic, "traffic_unit": "MB per launch (profiles/c2_mesh3d60_fp64_gemm_traffic.md)",
+                     "algorithmic_flops_per_launch": round(tm.gemm_flops / max(tm.gemm_launches, 1), 1)},
         "cpu_baseline": cpu,
         "numeric_breakdown": breakdown,
         "mixed_precision_varcoef": varcoef,
