@@ -163,7 +163,7 @@ def bench_batch(T, count=512, lo=16, hi=128, steps=3, seed=7):
         t0 = time.perf_counter()
         Bt = T.Batch(mats, precision=prec)
         t_setup = time.perf_counter() - t0
-        tuned = Bt.autotune(candidates=(64, 128, 256), reps=2)
+        tuned = Bt.autotune(reps=2)
         log(f"batch {name} autotune: {tuned}")
         n = Bt.n_total
         b = np.concatenate([T.random_uniform(m.n, 500 + q) for q, m in enumerate(mats)])
@@ -204,7 +204,8 @@ def bench_batch(T, count=512, lo=16, hi=128, steps=3, seed=7):
              "max_system_backward_error": float(max(bw)),
              "numeric_ms_values_from_host": round(1e3 * t_host, 3),
              "setup_s": round(t_setup, 2), "levels": Bt.n_levels,
-             "panel_width": tuned[0], "autotune_ms": {str(w): round(v, 3) for w, v in tuned[1].items()}}
+             "schedule": {"panel_width": tuned[0][0], "gemm_tile": tuned[0][1]},
+             "autotune_ms": {f"w{w}_t{t}": round(v, 3) for (w, t), v in tuned[1].items()}}
         if prec != T.FP64:
             r["refine_iterations"] = int(info.iterations)
         res[name] = r
@@ -274,7 +275,7 @@ def main():
         T.lib.taucs_amd_set_panel_width(args.panel_width)
         tuned_w, tune_ms = args.panel_width, {}
     else:
-        tuned_w, tune_ms = F.autotune(PA, T.FP64, candidates=(64, 128, 256, 512), reps=2)
+        tuned_w, tune_ms = F.autotune(PA, T.FP64, reps=2)
     log(f"autotune: panel width {tuned_w}, candidates (ms) {tune_ms}")
     B = np.stack([T.random_uniform(n, 1 + r) for r in range(k)], 1)
     dB = T.DeviceArray(B)
@@ -420,8 +421,9 @@ def main():
                                f"numeric factor + {k}-rhs supernodal solve (plain FP64 path)",
                    "n": n, "nrhs": k, "fund_flops": fi.fund_flops, "relax_flops": fi.relax_flops,
                    "n_supernodes": fi.n_live, "levels": fi.n_levels, "max_front": fi.max_front,
-                   "panel_width": tuned_w,
-                   "autotune_ms": {str(w): round(v, 3) for w, v in tune_ms.items()}},
+                   "schedule": {"panel_width": tuned_w[0] if isinstance(tuned_w, tuple) else tuned_w,
+                                "gemm_tile": tuned_w[1] if isinstance(tuned_w, tuple) else 64},
+                   "autotune_ms": {f"w{w}_t{t}": round(v, 3) for (w, t), v in tune_ms.items()}},
         "factor_ms": round(1e3 * tf_mean, 3),
         "solve_ms": round(1e3 * ts_mean, 3),
         "solves_per_s": round(k / ts_mean, 2),

@@ -117,16 +117,19 @@ void  taucs_amd_batch_free(void* B);
 
 /* ---- schedule shape and its autotuner ------------------------------------ */
 /* Large fronts are factored in outer panels of w columns (64-column POTRF /
-   TRSM steps inside, one trailing update of depth w per panel).  Default 256;
-   applies to schedules built afterwards. */
+   TRSM steps inside, one trailing update of depth w per panel), and the
+   trailing updates and SYRKs use GEMM tiles of edge 64 or 128.  Defaults
+   256 and 64; they apply to schedules built afterwards. */
 void  taucs_amd_set_panel_width(int w);
-/* Time the numeric factorization of A with each candidate panel width
-   (multiples of 64) and keep the fastest schedule in L, factored.  ms[i]:
-   best device time of candidate i over `reps` runs; *best_w: the choice. */
-int   taucs_amd_autotune(void* L, taucs_ccs_matrix* A, int precision, const int* cand,
-                         int ncand, int reps, double* ms, int* best_w);
-int   taucs_amd_batch_autotune(void* B, const int* cand, int ncand, int reps, double* ms,
-                               int* best_w);
+void  taucs_amd_set_gemm_tile(int edge);
+/* Time the numeric factorization of A with each candidate schedule (panel
+   width cand_w[i], multiple of 64; GEMM tile edge cand_tile[i], or NULL for
+   the default) and keep the fastest schedule in L, factored.  ms[i]: best
+   device time of candidate i over `reps` runs; *best: its index. */
+int   taucs_amd_autotune(void* L, taucs_ccs_matrix* A, int precision, const int* cand_w,
+                         const int* cand_tile, int ncand, int reps, double* ms, int* best);
+int   taucs_amd_batch_autotune(void* B, const int* cand_w, const int* cand_tile, int ncand,
+                               int reps, double* ms, int* best);
 
 /* hipGraph capture of the numeric schedule and of the solves (default on) */
 void  taucs_amd_set_graphs(int on);

@@ -102,10 +102,11 @@ _sigs = {
     "taucs_amd_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "taucs_amd_factor_llt_numeric_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "taucs_amd_batch_numeric_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
-    "taucs_amd_autotune": (C.c_int, [C.c_void_p, _ccs_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
-                                     C.c_void_p, C.c_void_p]),
-    "taucs_amd_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
-                                           C.c_void_p]),
+    "taucs_amd_set_gemm_tile": (None, [C.c_int]),
+    "taucs_amd_autotune": (C.c_int, [C.c_void_p, _ccs_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.c_int, C.c_void_p, C.c_void_p]),
+    "taucs_amd_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                           C.c_void_p, C.c_void_p]),
     "taucs_amd_get_op_profile": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 6),
     "taucs_amd_batch_factor_llt_mf": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int]),
     "taucs_amd_batch_numeric": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
@@ -353,15 +354,19 @@ class Factor:
         """Numeric factorization from values already in device memory."""
         return lib.taucs_amd_factor_llt_numeric_device(self.h, d_vals, precision)
 
-    def autotune(self, A, precision=FP64, candidates=(64, 128, 256, 512), reps=2):
-        """Pick the outer panel width by timing the numeric phase; L ends factored."""
-        cand = np.asarray(candidates, dtype=np.int32)
+    def autotune(self, A, precision=FP64, candidates=((64, 64), (128, 64), (256, 64), (256, 128),
+                                                       (512, 128)), reps=2):
+        """Pick the schedule shape (panel width, GEMM tile edge) by timing the
+        numeric phase; L ends factored.  Returns (best pair, {pair: ms})."""
+        cand = [(c, 0) if np.isscalar(c) else tuple(c) for c in candidates]
+        w = np.asarray([c[0] for c in cand], dtype=np.int32)
+        tl = np.asarray([c[1] for c in cand], dtype=np.int32)
         ms = np.zeros(len(cand))
         best = C.c_int()
-        if lib.taucs_amd_autotune(self.h, A.ptr, precision, _dptr(cand), len(cand), reps, _dptr(ms),
-                                  C.byref(best)):
+        if lib.taucs_amd_autotune(self.h, A.ptr, precision, _dptr(w), _dptr(tl), len(cand), reps,
+                                  _dptr(ms), C.byref(best)):
             raise RuntimeError("autotune failed")
-        return best.value, dict(zip(cand.tolist(), ms.tolist()))
+        return cand[best.value], dict(zip(cand, ms.tolist()))
 
     def op_profile(self):
         """Per-op kind, level, tasks, flops, bytes and ms of the last profiled numeric run."""
@@ -515,15 +520,18 @@ class Batch:
     def numeric_device(self, d_vals, precision=FP64):
         return lib.taucs_amd_batch_numeric_device(self.h, d_vals, precision)
 
-    def autotune(self, candidates=(64, 128, 256, 512), reps=2):
-        """Pick the batch schedule's outer panel width by timing; ends factored."""
-        cand = np.asarray(candidates, dtype=np.int32)
+    def autotune(self, candidates=((64, 64), (128, 64), (256, 64), (256, 128)), reps=2):
+        """Pick the batch schedule shape (panel width, GEMM tile edge) by timing;
+        the batch ends factored.  Returns (best pair, {pair: ms})."""
+        cand = [(c, 0) if np.isscalar(c) else tuple(c) for c in candidates]
+        w = np.asarray([c[0] for c in cand], dtype=np.int32)
+        tl = np.asarray([c[1] for c in cand], dtype=np.int32)
         ms = np.zeros(len(cand))
         best = C.c_int()
-        if lib.taucs_amd_batch_autotune(self.h, _dptr(cand), len(cand), reps, _dptr(ms),
+        if lib.taucs_amd_batch_autotune(self.h, _dptr(w), _dptr(tl), len(cand), reps, _dptr(ms),
                                         C.byref(best)):
             raise RuntimeError("batch autotune failed")
-        return best.value, dict(zip(cand.tolist(), ms.tolist()))
+        return cand[best.value], dict(zip(cand, ms.tolist()))
 
     def system_supernodes(self, q):
         f = Factor(lib.taucs_amd_batch_system_factor(self.h, q))

@@ -26,6 +26,7 @@ template <typename T> __global__ void k_potrf_wave(const FrontDesc*, const Panel
 template <typename T> __global__ void k_front_wave(const FrontDesc*, const int*, T*, T*, T*, int*);
 template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*, const T*);
 template <typename T> __global__ void k_trsm_tile(const GemmTask*, const T*, T*, T*, const T*);
+template <typename T> __global__ void k_gemm_tile128(const GemmTask*, const T*, T*, T*);
 __global__ void k_gemm_tile_bf16(const GemmTask*, const float*, float*, float*);
 template <typename T> __global__ void k_extend_add(const FrontDesc*, const EATask*, const int*, T*, T*);
 template <typename T> __global__ void k_solve_init(const FrontDesc*, const int*, const double*, int, T*, int);
@@ -60,6 +61,7 @@ struct Op {
   int level;
   int lane = 0;      // 1: may run on the side stream (the lookahead's trailing update)
   bool nojoin = false;  // touches only the current outer panel: need not wait for the side stream
+  int tile = 64;        // GEMM tile edge of the op's tasks (64 or 128)
 };
 
 // solve op kinds
@@ -91,6 +93,7 @@ struct tamd_dev {
   std::vector<int> lvl_ptr;      // fronts of level l: [lvl_ptr[l], lvl_ptr[l+1])
   int n_fronts = 0, ncol = 0, nnz = 0, n_levels = 0;
   int panel_w = 256;              // outer panel width of large fronts (multiple of 64)
+  int tile_big = 64;              // GEMM tile edge of the trailing updates and SYRKs
   long long W_per_rhs = 0;
   size_t struct_bytes = 0;       // device bytes of the schedule and A values
   double gemm_flops = 0, assembly_bytes = 0;
@@ -138,6 +141,8 @@ extern "C" void taucs_amd_set_panel_width(int w)
   w = (w + 63) / 64 * 64;
   g_panel_w = w < 64 ? 64 : (w > 1024 ? 1024 : w);
 }
+static int g_tile_big = 64;
+extern "C" void taucs_amd_set_gemm_tile(int edge) { g_tile_big = edge == 128 ? 128 : 64; }
 static int g_lookahead = 1;
 extern "C" void taucs_amd_set_lookahead(int on) { g_lookahead = on; }
 // Diagonal-block kernel: 1 = one wave with register broadcasts, 0 = 256
@@ -285,9 +290,11 @@ struct SysView {
 // Builds the device schedule of one or more independent systems.  Fronts of
 // level l of every system share the launches of level l; each system keeps
 // its own arenas (offsets), row numbering (shifted by xoff) and tree.
-static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = false, int panel_w = 0)
+static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = false, int panel_w = 0,
+                     int tile = 0)
 {
   D->panel_w = panel_w > 0 ? panel_w : g_panel_w;
+  D->tile_big = tile > 0 ? tile : g_tile_big;
   int nlev = 0, nf = 0;
   long long nnz_tot = 0;
   for (const SysView& v : sys) {
@@ -350,7 +357,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     for (int k = 0; k < P->n_live; k++) {
       int sn = P->order[k], g = gidx[q][k];
       fr[g].rows = (int)rows.size();
-      for (int e = 0; e < S->sn_up_size[sn]; e++) rows.push_back((int)(sys[q].xoff + S->sn_struct[sn][e]));
+      for (int e = 0; e < S->sn_up_size[sn]; e++) rows.push_back((int)(sys[q].xoff + P->drow[sn][e]));
     }
   }
   for (int g = 0; g < nf; g++) {
@@ -418,15 +425,15 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     // Lookahead: that update is split into the next outer panel's columns
     // (main stream) and the rest (side stream); the next panel's own steps
     // touch only its columns and overlap the rest.
-    auto tile = [&](const FrontDesc& d, int r0, int c0, int k0, int kw) {
+    auto tile = [&](const FrontDesc& d, int r0, int c0, int k0, int kw, int ts) {
       GemmTask g;
       g.a_off = d.offL + r0 + (long long)k0 * d.m;
       g.b_off = d.offL + c0 + (long long)k0 * d.m;
       g.c_off = d.offL + r0 + (long long)c0 * d.m;
       g.ld = d.m;
       g.ldc = d.m;
-      g.M = std::min(64, d.m - r0);
-      g.N = std::min(64, d.s - c0);
+      g.M = std::min(ts, d.m - r0);
+      g.N = std::min(ts, d.s - c0);
       g.K = kw;
       g.flags = (r0 == c0) ? 2 : 0;
       D->gemm_flops += tile_flops(g);
@@ -457,7 +464,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
             trsm.push_back(g);
           }
           for (int c0 = k0 + kb; c0 < pend; c0 += 64)
-            for (int r0 = c0; r0 < d.m; r0 += 64) gemm.push_back(tile(d, r0, c0, k0, kb));
+            for (int r0 = c0; r0 < d.m; r0 += 64) gemm.push_back(tile(d, r0, c0, k0, kb, 64));
         }
         Op o;
         if ((int)potrf.size() > p0) {
@@ -483,15 +490,20 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
         if (d.m <= SMALL_M) continue;
         const int K0 = os * W;
         if (K0 >= d.s) continue;
-        const int kw = std::min(W, d.s - K0);
-        for (int c0 = K0 + W; c0 < d.s; c0 += 64)
-          for (int r0 = c0; r0 < d.m; r0 += 64)
-            (c0 < K0 + 2 * W ? gemm : rest).push_back(tile(d, r0, c0, K0, kw));
+        const int kw = std::min(W, d.s - K0), ts = D->tile_big;
+        for (int c0 = K0 + W; c0 < d.s; c0 += ts)
+          for (int r0 = c0; r0 < d.m; r0 += ts)
+            (c0 < K0 + 2 * W ? gemm : rest).push_back(tile(d, r0, c0, K0, kw, ts));
       }
-      if ((int)gemm.size() > g0) D->ops.push_back({TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l});
+      if ((int)gemm.size() > g0) {
+        Op o = {TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l};
+        o.tile = D->tile_big;
+        D->ops.push_back(o);
+      }
       if (!rest.empty()) {
         Op o = {TAMD_OP_GEMM, (int)gemm.size(), (int)rest.size(), l};
         o.lane = 1;
+        o.tile = D->tile_big;
         D->ops.push_back(o);
         gemm.insert(gemm.end(), rest.begin(), rest.end());
       }
@@ -503,23 +515,28 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
         const FrontDesc& d = fr[k];
         int u = d.m - d.s;
         if (d.m <= SMALL_M || u == 0) continue;
-        for (int j0 = 0; j0 < u; j0 += 64)
-          for (int i0 = j0; i0 < u; i0 += 64) {
+        const int ts = D->tile_big;
+        for (int j0 = 0; j0 < u; j0 += ts)
+          for (int i0 = j0; i0 < u; i0 += ts) {
             GemmTask g;
             g.a_off = d.offL + d.s + i0;
             g.b_off = d.offL + d.s + j0;
             g.c_off = d.offU + i0 + (long long)j0 * u;
             g.ld = d.m;
             g.ldc = u;
-            g.M = std::min(64, u - i0);
-            g.N = std::min(64, u - j0);
+            g.M = std::min(ts, u - i0);
+            g.N = std::min(ts, u - j0);
             g.K = d.s;
             g.flags = 1 | (i0 == j0 ? 2 : 0);
             gemm.push_back(g);
             D->gemm_flops += tile_flops(g);
           }
       }
-      if ((int)gemm.size() > g0) D->ops.push_back({TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l});
+      if ((int)gemm.size() > g0) {
+        Op o = {TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l};
+        o.tile = D->tile_big;
+        D->ops.push_back(o);
+      }
     }
     // extend-add of this level's fronts into their parents, by sibling rank
     {
@@ -737,7 +754,7 @@ extern "C" int tamd_dev_batch_setup(tamd_factor* FB, int count, tamd_factor* con
     v[q].P = &F[q]->plan;
     v[q].xoff = xoff[q];
   }
-  if (dev_build(D, v, false, FB->panel_w) != 0) {
+  if (dev_build(D, v, false, FB->panel_w, FB->gemm_tile) != 0) {
     tamd_dev_free(FB);
     return -1;
   }
@@ -760,7 +777,7 @@ extern "C" int tamd_dev_setup(tamd_factor* F)
   one[0].S = &F->sym;
   one[0].P = &F->plan;
   one[0].xoff = 0;
-  if (dev_build(D, one, false, F->panel_w) != 0) {
+  if (dev_build(D, one, false, F->panel_w, F->gemm_tile) != 0) {
     tamd_dev_free(F);
     return -1;
   }
@@ -780,7 +797,7 @@ extern "C" long long taucs_amd_factor_device_bytes(void* vL, int precision, int 
   one[0].S = &F->sym;
   one[0].P = &F->plan;
   one[0].xoff = 0;
-  if (dev_build(&D, one, true, F->panel_w) != 0) return -1;
+  if (dev_build(&D, one, true, F->panel_w, F->gemm_tile) != 0) return -1;
   const long long elem = precision == TAMD_PREC_FP64 ? 8 : 4;
   const long long n = F->n, cap = std::max(nrhs, 16);
   long long b = (long long)D.struct_bytes + sizeof(int);
@@ -944,6 +961,9 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
         if (bf16upd)
           EMIT(lane, join, k_gemm_tile_bf16, dim3(op.count), dim3(256),
                (const GemmTask*)(D->d_gemm + op.start), (const float*)L, (float*)L, (float*)U);
+        else if (op.tile == 128)
+          EMIT(lane, join, k_gemm_tile128<T>, dim3(op.count), dim3(256),
+               (const GemmTask*)(D->d_gemm + op.start), (const T*)L, L, U);
         else
           EMIT(lane, join, k_gemm_tile<T>, dim3(op.count), dim3(256),
                (const GemmTask*)(D->d_gemm + op.start), (const T*)L, L, U, (const T*)Iv);

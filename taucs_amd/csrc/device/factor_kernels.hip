@@ -435,13 +435,23 @@ template <> struct Mfma<float> {
 
 // K is consumed in chunks of 32: the next chunk is fetched from global memory
 // into registers while the MFMAs run on the current chunk in LDS.
-template <typename T>
+// K-major LDS staging: a chunk of TK k-values is stored as As[row][k], and
+// the MFMA's k index is assigned so that lane group g (lane / 16) takes the
+// consecutive k = g*TK/4 .. g*TK/4 + TK/4 - 1 over the TK/4 steps of the chunk
+// (any assignment is valid as long as A and B use the same one).  Each lane
+// then reads its fragments for the whole chunk as contiguous 16-byte loads.
+// The next chunk is fetched into registers while the MFMAs run.  NBUF = 2:
+// it is stored into a second LDS buffer and one barrier per chunk separates
+// the phases; NBUF = 1 (half the LDS, so more resident workgroups): stored
+// after a second barrier.
+#define KP (TK + 2)
+template <typename T, int NBUF>
 __device__ __forceinline__ void gemm_tile(const GemmTask* __restrict__ tasks, const T* Lsrc, T* L,
                                           T* U, const T* __restrict__ Inv)
 {
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T As[TK][64 + TPAD];
-  __shared__ T Bs[TK][64 + TPAD];
+  __shared__ __align__(16) T As[NBUF][64][KP];
+  __shared__ __align__(16) T Bs[NBUF][64][KP];
   const GemmTask t = tasks[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
@@ -449,8 +459,11 @@ __device__ __forceinline__ void gemm_tile(const GemmTask* __restrict__ tasks, co
   const bool binv = (t.flags & 4) != 0;
   const T* B = binv ? Inv + t.b_off : Lsrc + t.b_off;
   acc_t acc[2][2];
+#pragma unroll
   for (int a = 0; a < 2; a++)
+#pragma unroll
     for (int b = 0; b < 2; b++)
+#pragma unroll
       for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
 
   const int li = tid & 63, lk = tid >> 6;  // loader: row li, k = lk + 4q
@@ -464,32 +477,48 @@ __device__ __forceinline__ void gemm_tile(const GemmTask* __restrict__ tasks, co
       rb[q] = (kin && li < t.N) ? (binv ? B[li * 64 + k] : B[li + (long long)k * t.ld]) : (T)0;
     }
   };
-  fetch(0);
-  for (int kc = 0; kc < t.K; kc += TK) {
+  auto stash = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < TK / 4; q++) {
-      As[lk + 4 * q][li] = ra[q];
-      Bs[lk + 4 * q][li] = rb[q];
+      As[buf][li][lk + 4 * q] = ra[q];
+      Bs[buf][li][lk + 4 * q] = rb[q];
     }
-    __syncthreads();
-    if (kc + TK < t.K) fetch(kc + TK);
+  };
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  const int c16 = lane & 15, g = lane >> 4;
+  const int nch = (t.K + TK - 1) / TK;
+  for (int ch = 0; ch < nch; ch++) {
+    const int buf = NBUF == 2 ? (ch & 1) : 0;
+    if (ch + 1 < nch) fetch((ch + 1) * TK);
+    T fa0[TK / 4], fa1[TK / 4], fb0[TK / 4], fb1[TK / 4];
 #pragma unroll
-    for (int kk = 0; kk < TK; kk += 4) {
-      const int kr = kk + (lane >> 4), c16 = lane & 15;
-      T a0 = As[kr][wr + c16], a1 = As[kr][wr + 16 + c16];
-      T b0 = Bs[kr][wc + c16], b1 = Bs[kr][wc + 16 + c16];
-      acc[0][0] = Mfma<T>::step(b0, a0, acc[0][0]);
-      acc[0][1] = Mfma<T>::step(b1, a0, acc[0][1]);
-      acc[1][0] = Mfma<T>::step(b0, a1, acc[1][0]);
-      acc[1][1] = Mfma<T>::step(b1, a1, acc[1][1]);
+    for (int s2 = 0; s2 < TK / 4; s2++) {
+      fa0[s2] = As[buf][wr + c16][g * (TK / 4) + s2];
+      fa1[s2] = As[buf][wr + 16 + c16][g * (TK / 4) + s2];
+      fb0[s2] = Bs[buf][wc + c16][g * (TK / 4) + s2];
+      fb1[s2] = Bs[buf][wc + 16 + c16][g * (TK / 4) + s2];
     }
+#pragma unroll
+    for (int s2 = 0; s2 < TK / 4; s2++) {
+      acc[0][0] = Mfma<T>::step(fb0[s2], fa0[s2], acc[0][0]);
+      acc[0][1] = Mfma<T>::step(fb1[s2], fa0[s2], acc[0][1]);
+      acc[1][0] = Mfma<T>::step(fb0[s2], fa1[s2], acc[1][0]);
+      acc[1][1] = Mfma<T>::step(fb1[s2], fa1[s2], acc[1][1]);
+    }
+    if (NBUF == 1) __syncthreads();
+    if (ch + 1 < nch) stash(NBUF == 2 ? (buf ^ 1) : 0);
     __syncthreads();
   }
 
   T* C = ((t.flags & 1) ? U : L) + t.c_off;
   const bool diag = (t.flags & 2) != 0;
+#pragma unroll
   for (int a = 0; a < 2; a++)
+#pragma unroll
     for (int b = 0; b < 2; b++)
+#pragma unroll
       for (int r = 0; r < 4; r++) {
         int i = wr + a * 16 + (lane & 15);
         int j = wc + b * 16 + Mfma<T>::drow(lane, r);
@@ -508,14 +537,91 @@ __global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ 
                                                    const T* Lsrc, T* L, T* U,
                                                    const T* __restrict__ Inv)
 {
-  gemm_tile<T>(tasks, Lsrc, L, U, Inv);
+  gemm_tile<T, 1>(tasks, Lsrc, L, U, Inv);
 }
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsm_tile(const GemmTask* __restrict__ tasks,
                                                    const T* Lsrc, T* L, T* U,
                                                    const T* __restrict__ Inv)
 {
-  gemm_tile<T>(tasks, Lsrc, L, U, Inv);
+  gemm_tile<T, 1>(tasks, Lsrc, L, U, Inv);
+}
+
+// 128 x 128 tile for the large trailing updates and SYRKs: 4 waves (2 x 2),
+// each a 64 x 64 sub-tile of 4 x 4 MFMA blocks, so every LDS fragment feeds
+// four MFMAs (twice the reuse of the 64 x 64 tile) and each tile reads half
+// the A/B bytes per flop.  K is staged in chunks of 16 with the next chunk
+// prefetched into registers.  Same task format (M, N <= 128) and epilogue.
+#define TK2 16
+#define TPAD2 4
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_tile128(const GemmTask* __restrict__ tasks,
+                                                      const T* Lsrc, T* L, T* U)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T As[TK2][128 + TPAD2];
+  __shared__ T Bs[TK2][128 + TPAD2];
+  const GemmTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const T* A = Lsrc + t.a_off;
+  const T* B = Lsrc + t.b_off;
+  acc_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
+
+  const int li = tid & 127, lk = tid >> 7;  // loader: row li, k = lk + 2q
+  T ra[TK2 / 2], rb[TK2 / 2];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int q = 0; q < TK2 / 2; q++) {
+      const int k = kc + lk + 2 * q;
+      const bool kin = k < t.K;
+      ra[q] = (kin && li < t.M) ? A[li + (long long)k * t.ld] : (T)0;
+      rb[q] = (kin && li < t.N) ? B[li + (long long)k * t.ld] : (T)0;
+    }
+  };
+  fetch(0);
+  for (int kc = 0; kc < t.K; kc += TK2) {
+#pragma unroll
+    for (int q = 0; q < TK2 / 2; q++) {
+      As[lk + 2 * q][li] = ra[q];
+      Bs[lk + 2 * q][li] = rb[q];
+    }
+    __syncthreads();
+    if (kc + TK2 < t.K) fetch(kc + TK2);
+#pragma unroll
+    for (int kk = 0; kk < TK2; kk += 4) {
+      const int kr = kk + (lane >> 4), c16 = lane & 15;
+      T af[4], bf[4];
+#pragma unroll
+      for (int a = 0; a < 4; a++) af[a] = As[kr][wr + 16 * a + c16];
+#pragma unroll
+      for (int b = 0; b < 4; b++) bf[b] = Bs[kr][wc + 16 * b + c16];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::step(bf[b], af[a], acc[a][b]);
+    }
+    __syncthreads();
+  }
+
+  T* C = ((t.flags & 1) ? U : L) + t.c_off;
+  const bool diag = (t.flags & 2) != 0;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = wr + a * 16 + (lane & 15);
+        const int j = wc + b * 16 + Mfma<T>::drow(lane, r);
+        if (i < t.M && j < t.N && (!diag || i >= j)) C[i + (long long)j * t.ldc] -= acc[a][b][r];
+      }
 }
 
 // Same tile, BF16 inputs with FP32 accumulation (v_mfma_f32_16x16x32_bf16):
@@ -527,9 +633,21 @@ __global__ __launch_bounds__(256) void k_gemm_tile_bf16(const GemmTask* __restri
 {
   __shared__ __bf16 As[64][32 + 8];  // [row][k]
   __shared__ __bf16 Bs[64][32 + 8];
-  const GemmTask t = tasks[blockIdx.x];
+  const GemmTask t0 = tasks[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  // a task larger than 64 x 64 (the 128-tile schedules) runs as 64 x 64 pieces
+  for (int si = 0; si < t0.M; si += 64)
+    for (int sj = 0; sj < t0.N; sj += 64) {
+  const bool tdiag = (t0.flags & 2) != 0;
+  if (tdiag && sj > si) continue;
+  GemmTask t = t0;
+  t.a_off += si;
+  t.b_off += sj;
+  t.c_off += si + (long long)sj * t0.ldc;
+  t.M = t0.M - si < 64 ? t0.M - si : 64;
+  t.N = t0.N - sj < 64 ? t0.N - sj : 64;
+  t.flags = (t0.flags & 1) | ((tdiag && si == sj) ? 2 : 0);
   const float* A = Lsrc + t.a_off;
   const float* B = Lsrc + t.b_off;
   f4_t acc[2][2];
@@ -571,6 +689,8 @@ __global__ __launch_bounds__(256) void k_gemm_tile_bf16(const GemmTask* __restri
         if (i < t.M && j < t.N && (!diag || i >= j))
           C[i + (long long)j * t.ldc] -= acc[a][b][r];
       }
+  __syncthreads();
+    }
 }
 
 // Extend-add: child update block entries (i >= j) added into the parent's
@@ -610,6 +730,7 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
   template __global__ void k_front_wave<T>(const FrontDesc*, const int*, T*, T*, T*, int*); \
   template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_trsm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
+  template __global__ void k_gemm_tile128<T>(const GemmTask*, const T*, T*, T*);          \
   template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*);
 TAMD_INST(double)
 TAMD_INST(float)

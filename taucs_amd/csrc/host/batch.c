@@ -195,32 +195,35 @@ int taucs_amd_batch_get_timing(void* vB, taucs_amd_numeric_timing* t)
 }
 
 /* The schedule autotuner (taucs_amd_autotune) for a batch: all systems share
-   one schedule, so one panel width is chosen for the whole batch. */
-int taucs_amd_batch_autotune(void* vB, const int* cand, int ncand, int reps, double* ms, int* best_w)
+   one schedule, so one schedule shape is chosen for the whole batch. */
+int taucs_amd_batch_autotune(void* vB, const int* cand_w, const int* cand_tile, int ncand, int reps,
+                             double* ms, int* best)
 {
   tamd_batch* B = (tamd_batch*)vB;
-  int c, best = -1;
+  int c, bi = -1;
   double best_ms = 1e300;
-  if (!B || !cand || ncand <= 0 || !B->all.has_numeric) return -1;
+  if (!B || !cand_w || ncand <= 0 || !B->all.has_numeric) return -1;
   for (c = 0; c < ncand; c++) {
     double t;
-    if (cand[c] < 64 || cand[c] % 64) return -1;
+    if (cand_w[c] < 64 || cand_w[c] % 64) return -1;
     tamd_dev_free(&B->all);
     B->all.has_numeric = 0;
-    B->all.panel_w = cand[c];
+    B->all.panel_w = cand_w[c];
+    B->all.gemm_tile = cand_tile ? cand_tile[c] : 0;
     if (tamd_dev_batch_setup(&B->all, B->count, B->sys, B->xoff) != 0
         || tamd_dev_time_numeric(&B->all, B->vals, reps, &t) != 0)
       return -1;
     if (ms) ms[c] = t;
-    if (t < best_ms) { best_ms = t; best = c; }
+    if (t < best_ms) { best_ms = t; bi = c; }
   }
   tamd_dev_free(&B->all);
-  B->all.panel_w = cand[best];
+  B->all.panel_w = cand_w[bi];
+  B->all.gemm_tile = cand_tile ? cand_tile[bi] : 0;
   if (tamd_dev_batch_setup(&B->all, B->count, B->sys, B->xoff) != 0
       || tamd_dev_numeric_values(&B->all, B->vals) != 0)
     return -1;
   B->all.has_numeric = 1;
-  if (best_w) *best_w = cand[best];
+  if (best) *best = bi;
   return 0;
 }
 

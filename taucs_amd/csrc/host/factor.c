@@ -223,25 +223,30 @@ taucs_ccs_matrix* taucs_dsupernodal_factor_to_ccs(void* vL)
   if (!F) return NULL;
   S = &F->sym;
   n = F->n;
+  int* dpos;  /* global row -> position in the current front's device row order */
   h = download_panels(F);
   if (!h) return NULL;
   len = (int*)tamd_calloc((size_t)n + 1, sizeof(int));
-  if (!len) { tamd_free(h); return NULL; }
+  dpos = (int*)tamd_malloc(sizeof(int) * ((size_t)n + 1));
+  if (!len || !dpos) { tamd_free(len); tamd_free(dpos); tamd_free(h); return NULL; }
+  /* entries go out in the reference's row order (sn_struct), read from the
+     device panel's row order */
   for (sn = 0; sn < S->n_sn; sn++) {
     int s = S->sn_size[sn], m = S->sn_up_size[sn];
     const double* P;
-    if (s == 0) continue;
+    if (s == 0 || !F->plan.drow[sn]) continue;
     P = h + F->plan.off_L[sn];
+    for (ip = 0; ip < m; ip++) dpos[F->plan.drow[sn][ip]] = ip;
     for (jp = 0; jp < s; jp++) {
       j = S->sn_struct[sn][jp];
       len[j] = 0;
       for (ip = jp; ip < m; ip++)
-        if (P[(int64_t)jp * m + ip] != 0.0) { len[j]++; nnz++; }
+        if (P[(int64_t)jp * m + dpos[S->sn_struct[sn][ip]]] != 0.0) { len[j]++; nnz++; }
     }
   }
-  if (nnz > 0x7fffffff) { tamd_free(len); tamd_free(h); return NULL; }
+  if (nnz > 0x7fffffff) { tamd_free(len); tamd_free(dpos); tamd_free(h); return NULL; }
   C = taucs_ccs_create(n, n, (int)nnz, TAUCS_DOUBLE);
-  if (!C) { tamd_free(len); tamd_free(h); return NULL; }
+  if (!C) { tamd_free(len); tamd_free(dpos); tamd_free(h); return NULL; }
   C->flags = TAUCS_DOUBLE | TAUCS_TRIANGULAR | TAUCS_LOWER;
   C->colptr[0] = 0;
   for (j = 0; j < n; j++) C->colptr[j + 1] = C->colptr[j] + len[j];
@@ -250,12 +255,14 @@ taucs_ccs_matrix* taucs_dsupernodal_factor_to_ccs(void* vL)
     const double* P;
     if (s == 0) continue;
     P = h + F->plan.off_L[sn];
+    if (!F->plan.drow[sn]) continue;
+    for (ip = 0; ip < m; ip++) dpos[F->plan.drow[sn][ip]] = ip;
     for (jp = 0; jp < s; jp++) {
       int next;
       j = S->sn_struct[sn][jp];
       next = C->colptr[j];
       for (ip = jp; ip < m; ip++) {
-        double v = P[(int64_t)jp * m + ip];
+        double v = P[(int64_t)jp * m + dpos[S->sn_struct[sn][ip]]];
         if (v == 0.0) continue;
         C->rowind[next] = S->sn_struct[sn][ip];
         C->values.d[next] = v;
@@ -264,6 +271,7 @@ taucs_ccs_matrix* taucs_dsupernodal_factor_to_ccs(void* vL)
     }
   }
   tamd_free(len);
+  tamd_free(dpos);
   tamd_free(h);
   return C;
 }
@@ -355,34 +363,37 @@ int taucs_amd_factor_get_schedule(void* vL, int* sn_level, int* sn_parent, long 
 }
 
 /* Autotuner of the schedule shape: the numeric phase is timed with each
-   candidate outer panel width (the trailing-update depth of the large
-   fronts), and the fastest schedule is kept in L, factored from A.
-   cand: candidate widths (multiples of 64); ms[i] receives candidate i's
-   best time over `reps` timed runs. */
-int taucs_amd_autotune(void* vL, taucs_ccs_matrix* A, int precision, const int* cand, int ncand,
-                       int reps, double* ms, int* best_w)
+   candidate schedule (outer panel width cand_w[i], multiple of 64, and tile
+   edge cand_tile[i] of the large GEMMs, 64 or 128; cand_tile may be NULL for
+   the default), and the fastest schedule is kept in L, factored from A.
+   ms[i] receives candidate i's best time over `reps` timed runs; *best its
+   index. */
+int taucs_amd_autotune(void* vL, taucs_ccs_matrix* A, int precision, const int* cand_w,
+                       const int* cand_tile, int ncand, int reps, double* ms, int* best)
 {
   tamd_factor* F = (tamd_factor*)vL;
-  int c, best = -1;
+  int c, bi = -1;
   double best_ms = 1e300;
-  if (!F || !A || !cand || ncand <= 0 || check_input(A, "taucs_amd_autotune") != 0) return -1;
+  if (!F || !A || !cand_w || ncand <= 0 || check_input(A, "taucs_amd_autotune") != 0) return -1;
   if (A->n != F->n || tamd_pattern_hash(A) != F->plan.a_pattern_hash) return -1;
   F->precision = precision;
   for (c = 0; c < ncand; c++) {
     double t;
-    if (cand[c] < 64 || cand[c] % 64) return -1;
+    if (cand_w[c] < 64 || cand_w[c] % 64) return -1;
     tamd_dev_free(F);
     F->has_numeric = 0;
-    F->panel_w = cand[c];
+    F->panel_w = cand_w[c];
+    F->gemm_tile = cand_tile ? cand_tile[c] : 0;
     if (tamd_dev_setup(F) != 0 || tamd_dev_time_numeric(F, A->values.d, reps, &t) != 0) return -1;
     if (ms) ms[c] = t;
-    if (t < best_ms) { best_ms = t; best = c; }
+    if (t < best_ms) { best_ms = t; bi = c; }
   }
   tamd_dev_free(F);
-  F->panel_w = cand[best];
+  F->panel_w = cand_w[bi];
+  F->gemm_tile = cand_tile ? cand_tile[bi] : 0;
   if (tamd_dev_setup(F) != 0 || tamd_dev_numeric(F, A) != 0) return -1;
   F->has_numeric = 1;
-  if (best_w) *best_w = cand[best];
+  if (best) *best = bi;
   return 0;
 }
 

@@ -63,6 +63,9 @@ typedef struct {
                              the front's own level; it is dead after its own level */
   int64_t  L_elems;
   int64_t  U_elems;
+  int**    drow;          /* [n_sn] device row order of each live front: its s columns as
+                             in sn_struct, then its update rows sorted by their position in
+                             the parent's device order (so extend-add indices increase) */
   int*     rel_ptr;       /* [n_sn+1] offsets into rel */
   int*     rel;           /* child update row i -> position in parent front */
   int*     sib_rank;      /* [n_sn] rank among same-level siblings */
@@ -87,6 +90,7 @@ typedef struct tamd_factor {
   int            has_numeric;
   double         t_symbolic, t_numeric; /* seconds */
   int            panel_w;    /* outer panel width of the schedule (0: library default) */
+  int            gemm_tile;  /* tile edge of the large GEMMs, 64 or 128 (0: library default) */
 } tamd_factor;
 
 /* ---- host symbolic layer (symbolic.c, mmd.c) ---- */

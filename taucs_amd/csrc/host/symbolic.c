@@ -417,6 +417,11 @@ void tamd_plan_free(tamd_plan* P)
   tamd_free(P->off_L);
   tamd_free(P->off_U);
   tamd_free(P->u_start);
+  if (P->drow) {
+    int sn;
+    for (sn = 0; sn < P->n_sn; sn++) tamd_free(P->drow[sn]);
+  }
+  tamd_free(P->drow);
   tamd_free(P->rel_ptr);
   tamd_free(P->rel);
   tamd_free(P->sib_rank);
@@ -441,7 +446,7 @@ int tamd_plan_amap(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_plan*
     int m;
     sn = P->order[k];
     m = S->sn_up_size[sn];
-    for (ip = 0; ip < m; ip++) pos[S->sn_struct[sn][ip]] = ip;
+    for (ip = 0; ip < m; ip++) pos[P->drow[sn][ip]] = ip;
     for (jj = 0; jj < S->sn_size[sn]; jj++) {
       int col = S->sn_struct[sn][jj];
       for (ip = A->colptr[col]; ip < A->colptr[col + 1]; ip++) {
@@ -456,6 +461,41 @@ int tamd_plan_amap(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_plan*
   P->a_pattern_hash = tamd_pattern_hash(A);
   tamd_free(pos);
   return 0;
+}
+
+/* Sort v[0..len) by key[v[i]] (keys distinct): insertion sort for short
+   lists, else a heap sort; deterministic. */
+static void sift_down(int* v, int start, int end, const int* key)
+{
+  int root = start;
+  while (2 * root + 1 < end) {
+    int child = 2 * root + 1, sw = root;
+    if (key[v[sw]] < key[v[child]]) sw = child;
+    if (child + 1 < end && key[v[sw]] < key[v[child + 1]]) sw = child + 1;
+    if (sw == root) return;
+    { int t = v[root]; v[root] = v[sw]; v[sw] = t; }
+    root = sw;
+  }
+}
+
+static void sort_by_key(int* v, int len, const int* key)
+{
+  int i, j;
+  if (len < 32) {
+    for (i = 1; i < len; i++) {
+      int x = v[i];
+      for (j = i; j > 0 && key[v[j - 1]] > key[x]; j--) v[j] = v[j - 1];
+      v[j] = x;
+    }
+    return;
+  }
+  for (i = len / 2 - 1; i >= 0; i--) sift_down(v, i, len, key);
+  for (i = len - 1; i > 0; i--) {
+    int t = v[0];
+    v[0] = v[i];
+    v[i] = t;
+    sift_down(v, 0, i, key);
+  }
 }
 
 /* Offsets of the update blocks U_sn (u x u, u = m - s) in one arena whose
@@ -633,6 +673,25 @@ int tamd_plan_build(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_plan
   }
   memcpy(P->order, stack, sizeof(int) * (size_t)nlive);
 
+  /* device row order: parents before children (decreasing level), each
+     front's update rows sorted by their position in the parent's order */
+  P->drow = (int**)tamd_calloc((size_t)nsn + 1, sizeof(int*));
+  if (!P->drow) goto fail;
+  for (k = nlive - 1; k >= 0; k--) {
+    int m, s, ip, p;
+    sn = P->order[k];
+    s = S->sn_size[sn];
+    m = S->sn_up_size[sn];
+    p = P->sn_parent[sn];
+    P->drow[sn] = (int*)tamd_malloc(sizeof(int) * (size_t)(m > 0 ? m : 1));
+    if (!P->drow[sn]) goto fail;
+    memcpy(P->drow[sn], S->sn_struct[sn], sizeof(int) * (size_t)m);
+    if (p < nsn && m > s) {
+      for (ip = 0; ip < S->sn_up_size[p]; ip++) pos[P->drow[p][ip]] = ip;
+      sort_by_key(P->drow[sn] + s, m - s, pos);
+    }
+  }
+
   /* panels, update blocks, relative indices */
   P->max_front = 0;
   for (sn = 0; sn <= nsn; sn++) { P->off_L[sn] = -1; P->off_U[sn] = -1; }
@@ -662,9 +721,9 @@ int tamd_plan_build(const tamd_symbolic* S, const taucs_ccs_matrix* A, tamd_plan
       p = P->sn_parent[sn];
       P->rel_ptr[sn] = r;
       if (p >= nsn) continue;
-      for (ip = 0; ip < S->sn_up_size[p]; ip++) pos[S->sn_struct[p][ip]] = ip;
+      for (ip = 0; ip < S->sn_up_size[p]; ip++) pos[P->drow[p][ip]] = ip;
       for (ip = S->sn_size[sn]; ip < S->sn_up_size[sn]; ip++)
-        P->rel[r++] = pos[S->sn_struct[sn][ip]];
+        P->rel[r++] = pos[P->drow[sn][ip]];
     }
     P->rel_ptr[nsn] = r;
   }
