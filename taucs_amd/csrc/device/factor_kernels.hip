@@ -694,28 +694,32 @@ __global__ __launch_bounds__(256) void k_gemm_tile_bf16(const GemmTask* __restri
 }
 
 // Extend-add: child update block entries (i >= j) added into the parent's
-// panel or update block at the relative positions, kept in the lower part.
+// panel or update block at the relative positions (src/taucs_sn_llt.c:1200-1240).
 template <typename T>
 __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict__ fr,
                                                     const EATask* __restrict__ tasks,
                                                     const int* __restrict__ rel,
                                                     T* __restrict__ L, T* __restrict__ U)
 {
+  // The child's update rows are ordered by their parent positions, so
+  // r[i] > r[j] for i > j: entry (i, j) goes to parent (r[i], r[j]), below the
+  // diagonal, and one child column lands in one parent column.  Each wave
+  // takes every fourth column of the task; its lanes walk the column.
   const EATask t = tasks[blockIdx.x];
   const FrontDesc c = fr[t.child];
   const FrontDesc p = fr[c.parent];
   const int uc = c.m - c.s, sp = p.s, mp = p.m, up = mp - sp;
   const T* Uc = U + c.offU;
   const int* r = rel + c.rel;
-  for (int j = t.j0; j < t.j1; j++) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = t.j0 + wave; j < t.j1; j += 4) {
     const int pj = r[j];
-    for (int i = j + threadIdx.x; i < uc; i += blockDim.x) {
-      const int pi = r[i];
-      const int row = pi > pj ? pi : pj, col = pi > pj ? pj : pi;
-      const T v = Uc[i + (long long)j * uc];
-      if (col < sp) L[p.offL + row + (long long)col * mp] += v;
-      else U[p.offU + (row - sp) + (long long)(col - sp) * up] += v;
-    }
+    const T* src = Uc + (long long)j * uc;
+    // parent column pj: panel column (rows 0..mp) or update-block column (rows sp..mp)
+    T* base = pj < sp ? L : U;
+    const long long col0 = pj < sp ? p.offL + (long long)pj * mp
+                                   : p.offU + (long long)(pj - sp) * up - sp;
+    for (int i = j + lane; i < uc; i += 64) base[col0 + r[i]] += src[i];
   }
 }
 
