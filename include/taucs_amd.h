@@ -131,7 +131,7 @@ int   taucs_amd_autotune(void* L, taucs_ccs_matrix* A, int precision, const int*
 int   taucs_amd_batch_autotune(void* B, const int* cand_w, const int* cand_tile, int ncand,
                                int reps, double* ms, int* best);
 
-/* hipGraph capture of the numeric schedule and of the solves (default on) */
+/* Replay the numeric schedule and the solves as hipGraphs built node by node (default on) */
 void  taucs_amd_set_graphs(int on);
 /* Panel lookahead: the trailing update of a panel step runs on a second stream
    while the next panel is factored (default on; results are identical). */
@@ -139,6 +139,11 @@ void  taucs_amd_set_lookahead(int on);
 /* Diagonal-block POTRF kernel: 1 = one wave, register broadcasts (default);
    0 = 256 threads through LDS. */
 void  taucs_amd_set_potrf_kernel(int wave);
+/* Triangular sweeps of the solve: 1 = one launch per tree level, the block
+   chain of each front followed inside the launch through per-block flags
+   (default); 0 = one launch per 64-column block step.  Applies to schedules
+   built afterwards. */
+void  taucs_amd_set_solve_kernel(int chained);
 
 /* ---- device buffers (for right-hand sides kept in HBM) ------------------- */
 void* taucs_amd_device_malloc(size_t bytes);

@@ -29,7 +29,7 @@ template <typename T> __global__ void k_trsm_tile(const GemmTask*, const T*, T*,
 template <typename T> __global__ void k_gemm_tile128(const GemmTask*, const T*, T*, T*);
 __global__ void k_gemm_tile_bf16(const GemmTask*, const float*, float*, float*);
 template <typename T> __global__ void k_extend_add(const FrontDesc*, const EATask*, const int*, T*, T*);
-template <typename T> __global__ void k_solve_init(const FrontDesc*, const int*, const double*, int, T*, int);
+template <typename T> __global__ void k_solve_init(const FrontDesc*, const PanelTask*, int, const int*, const double*, int, T*, int);
 template <typename T> __global__ void k_fwd_step(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, T*, T*, int, int);
 template <typename T> __global__ void k_fwd_rect(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, int, T*, int);
 template <typename T> __global__ void k_fwd_rect_reduce(const FrontDesc*, const PanelTask*, const T*, T*, int);
@@ -38,6 +38,11 @@ template <typename T> __global__ void k_solve_ext(const FrontDesc*, const PanelT
 template <typename T> __global__ void k_bwd_gather(const FrontDesc*, const PanelTask*, const int*, const double*, int, T*, int);
 template <typename T> __global__ void k_bwd_cols(const FrontDesc*, const PanelTask*, const T*, const T*, T*, int);
 template <typename T> __global__ void k_bwd_step(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, T*, int, double*, int, int);
+template <typename T> __global__ void k_sweep_clear(T*, double*, int, int, int, int*, int);
+template <typename T> __global__ void k_fwd_chain(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, T*, T*, int, int*, int*, int);
+template <typename T> __global__ void k_bwd_chain(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, const T*, int, double*, int, int*, int*, int);
+template <typename T> __global__ void k_fwd_small(const FrontDesc*, const PanelTask*, int, const int*, const T*, const T*, T*, T*, int, int);
+template <typename T> __global__ void k_bwd_small(const FrontDesc*, const PanelTask*, int, const int*, const T*, const T*, const T*, int, double*, int, int);
 }  // namespace tamd
 
 using namespace tamd;
@@ -65,7 +70,8 @@ struct Op {
 };
 
 // solve op kinds
-enum { SV_INIT, SV_FSTEP, SV_FRECT, SV_FRED, SV_EXT, SV_GATHER, SV_BCOLS, SV_BRED, SV_BSTEP };
+enum { SV_INIT, SV_FSTEP, SV_FRECT, SV_FRED, SV_EXT, SV_GATHER, SV_BCOLS, SV_BRED, SV_BSTEP,
+       SV_CLEAR, SV_FCHAIN, SV_BCHAIN, SV_FSMALL, SV_BSMALL };
 
 struct tamd_dev {
   hipStream_t stream = nullptr;
@@ -85,7 +91,11 @@ struct tamd_dev {
   EATask* d_ea = nullptr;
   ZeroTask* d_zero = nullptr;
   PanelTask* d_sv_tasks = nullptr;  // solve tasks
-  std::vector<Op> sv_ops;
+  std::vector<Op> sv_ops;           // block-step sweeps (one launch per block step)
+  std::vector<Op> sv_chain_ops;     // chained sweeps (one launch per level); Op.tile = ticket
+  int solve_chain = 1;
+  int* d_sync = nullptr;            // the chained launches' tickets, then the error word
+  long long sync_n = 0;             // tickets (cleared at the start of every sweep)
   int* d_info = nullptr;
   std::vector<Op> ops;
   std::vector<double> op_flops, op_bytes;  // algorithmic work of each op
@@ -134,6 +144,8 @@ struct tamd_dev {
 };
 
 static int g_graphs = 1;
+static int g_solve_chain = 1;
+extern "C" void taucs_amd_set_solve_kernel(int chained) { g_solve_chain = chained ? 1 : 0; }
 extern "C" void taucs_amd_set_graphs(int on) { g_graphs = on; }
 static int g_panel_w = 256;
 extern "C" void taucs_amd_set_panel_width(int w)
@@ -243,6 +255,8 @@ extern "C" void tamd_dev_free_numeric(tamd_factor* F)
   free_ptr(D->W);
   free_ptr(D->Y);
   free_ptr(D->Part);
+  free_ptr(D->d_sync);
+  D->d_sync = nullptr;
   D->L = D->U = D->Inv = D->W = D->Y = D->Part = nullptr;
   D->W_rhs = 0;
   D->prec = -1;
@@ -569,11 +583,14 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
   // every op is a list of PanelTasks (front, k0, kb, r0, nr)
   std::vector<PanelTask> svt;
   D->sv_ops.clear();
-  D->sv_ops.push_back({SV_INIT, 0, nf, -1});
+  for (int k = 0; k < nf; k++)  // k_solve_init: 64-row pieces of every front
+    for (int r0 = 0; r0 < fr[k].m; r0 += 64) svt.push_back({k, 0, 0, r0, std::min(64, fr[k].m - r0)});
+  const int n_init = (int)svt.size();
+  D->sv_ops.push_back({SV_INIT, 0, n_init, -1});
   auto push_tasks = [&](int kind, size_t t0, int l) {
     if (svt.size() > t0) D->sv_ops.push_back({kind, (int)t0, (int)(svt.size() - t0), l});
   };
-  const int SB = 64, CH = 512;
+  const int SB = 64, CH = 512, CHB = 128;
   D->part_slots = 0;
   for (int l = 0; l < nlev; l++) {
     int b0 = lvl[l], b1 = lvl[l + 1];
@@ -660,6 +677,104 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       push_tasks(SV_BSTEP, t0, l);
     }
   }
+
+  // chained sweeps: per level, the fronts with m <= 64 one wave each
+  // (k_fwd_small / k_bwd_small); the others through k_fwd_chain /
+  // k_bwd_chain, in one ticketed launch whose task order puts every task
+  // after the tasks it waits for (forward: the block tiles round-robin by
+  // block index, then the sub-diagonal tiles; backward: the blocks from the
+  // last one), single-block fronts of the backward without tickets.  Op.tile
+  // holds the launch's ticket index (-1: none).
+  D->sv_chain_ops.clear();
+  D->sv_chain_ops.push_back({SV_CLEAR, 0, 0, -1});
+  D->sv_chain_ops.push_back({SV_INIT, 0, n_init, -1});
+  int ntickets = 0;
+  auto push_chain = [&](int kind, size_t t0, int l, bool ticketed) {
+    if (svt.size() <= t0) return;
+    Op o{kind, (int)t0, (int)(svt.size() - t0), l};
+    o.tile = ticketed ? ntickets++ : -1;
+    D->sv_chain_ops.push_back(o);
+  };
+  auto nblk = [&](int k) { return (fr[k].s + SB - 1) / SB; };
+  for (int l = 0; l < nlev; l++) {
+    const int b0 = lvl[l], b1 = lvl[l + 1];
+    size_t t0 = svt.size();
+    for (int k = b0; k < b1; k++)
+      if (fr[k].m <= 64) svt.push_back({k, 0, 0, 0, 0});
+    push_chain(SV_FSMALL, t0, l, false);
+    t0 = svt.size();
+    int maxb = 0;
+    for (int k = b0; k < b1; k++)
+      if (fr[k].m > 64) maxb = std::max(maxb, nblk(k));
+    for (int b = 0; b < maxb; b++)
+      for (int k = b0; k < b1; k++)
+        if (fr[k].m > 64 && b < nblk(k)) svt.push_back({k, b, 0, 0, 0});
+    for (int k = b0; k < b1; k++)
+      if (fr[k].m > 64)
+        for (int tt = nblk(k); tt < (fr[k].m + 63) / 64; tt++) svt.push_back({k, tt, 0, 0, 0});
+    push_chain(SV_FCHAIN, t0, l, true);
+    int maxrank = -1;
+    for (int k = b0; k < b1; k++)
+      if (fr[k].parent >= 0 && fr[k].m > fr[k].s) maxrank = std::max(maxrank, sibr[k]);
+    for (int rk = 0; rk <= maxrank; rk++) {
+      t0 = svt.size();
+      for (int k = b0; k < b1; k++) {
+        int u = fr[k].m - fr[k].s;
+        if (fr[k].parent < 0 || u == 0 || sibr[k] != rk) continue;
+        for (int r0 = 0; r0 < u; r0 += 256) svt.push_back({k, 0, 0, r0, std::min(256, u - r0)});
+      }
+      push_chain(SV_EXT, t0, l, false);
+    }
+  }
+  for (int l = nlev - 1; l >= 0; l--) {
+    const int b0 = lvl[l], b1 = lvl[l + 1];
+    size_t t0 = svt.size();
+    // fronts with m > 64: y -= L21^T x2 first, split over row chunks
+    // (k_bwd_gather, k_bwd_cols, k_bwd_cols_reduce), so the chained launch
+    // only follows the diagonal block chain
+    for (int k = b0; k < b1; k++) {
+      const int u = fr[k].m - fr[k].s;
+      if (fr[k].m <= 64) continue;
+      for (int r0 = 0; r0 < u; r0 += 256) svt.push_back({k, 0, 0, r0, std::min(256, u - r0)});
+    }
+    push_chain(SV_GATHER, t0, l, false);
+    {
+      t0 = svt.size();
+      std::vector<PanelTask> red;
+      for (int k = b0; k < b1; k++) {
+        if (fr[k].m <= 64 || fr[k].m == fr[k].s) continue;
+        for (int c0 = 0; c0 < fr[k].s; c0 += 64) {
+          int nc = std::min(64, fr[k].s - c0), slot0 = (int)(svt.size() - t0), cnt = 0;
+          for (int i0 = fr[k].s; i0 < fr[k].m; i0 += CHB, cnt++)
+            svt.push_back({k, i0, std::min(CHB, fr[k].m - i0), c0, nc});
+          red.push_back({k, slot0, cnt, c0, nc});
+        }
+      }
+      D->part_slots = std::max(D->part_slots, (long long)(svt.size() - t0));
+      push_chain(SV_BCOLS, t0, l, false);
+      t0 = svt.size();
+      svt.insert(svt.end(), red.begin(), red.end());
+      push_chain(SV_BRED, t0, l, false);
+    }
+    t0 = svt.size();
+    for (int k = b0; k < b1; k++)
+      if (fr[k].m <= 64) svt.push_back({k, 0, 0, 0, 0});
+    push_chain(SV_BSMALL, t0, l, false);
+    t0 = svt.size();
+    for (int k = b0; k < b1; k++)
+      if (fr[k].m > 64 && nblk(k) == 1) svt.push_back({k, 0, 0, 0, 0});
+    push_chain(SV_BCHAIN, t0, l, false);
+    t0 = svt.size();
+    int maxb = 0;
+    for (int k = b0; k < b1; k++)
+      if (nblk(k) > 1) maxb = std::max(maxb, nblk(k));
+    for (int q = 0; q < maxb; q++)
+      for (int k = b0; k < b1; k++)
+        if (nblk(k) > 1 && q < nblk(k)) svt.push_back({k, nblk(k) - 1 - q, 0, 0, 0});
+    push_chain(SV_BCHAIN, t0, l, true);
+  }
+  D->sync_n = ntickets;
+  D->solve_chain = g_solve_chain;
 
   // concatenated A pattern: column j of system q is global column xoff_q + j
   long long ncol = 0;
@@ -803,6 +918,7 @@ extern "C" long long taucs_amd_factor_device_bytes(void* vL, int precision, int 
   long long b = (long long)D.struct_bytes + sizeof(int);
   b += elem * (std::max(D.L_elems, 1LL) + std::max(D.U_elems, 1LL) + std::max(D.I_elems, 1LL));
   b += elem * (D.W_per_rhs * cap + n * cap + 64LL * 16 * std::max(D.part_slots, 1LL));
+  b += (long long)sizeof(int) * (D.sync_n + 1);
   b += 2LL * 8 * n * std::max(nrhs, 1);
   if (refine) {
     const long long full = 2LL * F->plan.a_nnz;
@@ -1177,12 +1293,15 @@ static int emit_solve(tamd_dev* D, Emitter& E, int n, int nrhs, double* dX, int 
   const T* Iv = (const T*)D->Inv;
   const FrontDesc* fr = D->d_fronts;
   const int* rows = D->d_rows;
-  for (const Op& op : D->sv_ops) {
+  int* tickets = D->d_sync;
+  int* err = D->d_sync + D->sync_n;
+  for (const Op& op : D->solve_chain ? D->sv_chain_ops : D->sv_ops) {
     const PanelTask* tk = D->d_sv_tasks + op.start;
     const dim3 grid(op.count);
     switch (op.kind) {
       case SV_INIT:
-        EMIT(0, true, k_solve_init<T>, grid, dim3(128), fr, rows, dB, ldb, W, nrhs);
+        EMIT(0, true, k_solve_init<T>, dim3((op.count + 3) / 4), dim3(256), fr, tk, op.count, rows,
+             dB, ldb, W, nrhs);
         break;
       case SV_FSTEP:
         EMIT(0, true, k_fwd_step<T>, grid, dim3(256), fr, tk, rows, L, Iv, W, Y, n, nrhs);
@@ -1208,6 +1327,26 @@ static int emit_solve(tamd_dev* D, Emitter& E, int n, int nrhs, double* dX, int 
       case SV_BSTEP:
         EMIT(0, true, k_bwd_step<T>, grid, dim3(256), fr, tk, rows, L, Iv, Y, n, dX, ldx, nrhs);
         break;
+      case SV_CLEAR:
+        EMIT(0, true, k_sweep_clear<T>, dim3(std::min(grid_for((long long)n * nrhs, 256), 4096u)),
+             dim3(256), Y, dX, n, ldx, nrhs, tickets, (int)D->sync_n);
+        break;
+      case SV_FCHAIN:
+        EMIT(0, true, k_fwd_chain<T>, grid, dim3(256), fr, tk, rows, L, Iv, W, Y, n,
+             op.tile >= 0 ? tickets + op.tile : (int*)nullptr, err, nrhs);
+        break;
+      case SV_BCHAIN:
+        EMIT(0, true, k_bwd_chain<T>, grid, dim3(256), fr, tk, rows, L, Iv, (const T*)Y, n, dX, ldx,
+             op.tile >= 0 ? tickets + op.tile : (int*)nullptr, err, nrhs);
+        break;
+      case SV_FSMALL:
+        EMIT(0, true, k_fwd_small<T>, dim3((op.count + 3) / 4), dim3(256), fr, tk, op.count, rows, L,
+             Iv, W, Y, n, nrhs);
+        break;
+      case SV_BSMALL:
+        EMIT(0, true, k_bwd_small<T>, dim3((op.count + 3) / 4), dim3(256), fr, tk, op.count, rows, L,
+             Iv, (const T*)Y, n, dX, ldx, nrhs);
+        break;
     }
   }
   return 0;
@@ -1232,6 +1371,10 @@ static int run_solve(tamd_factor* F, int nrhs, double* dX, int ldx, const double
         || dev_malloc(&D->Part, sizeof(T) * 64 * 16 * (size_t)std::max(D->part_slots, 1LL)) != 0)
       return -1;
     D->W_rhs = cap;
+  }
+  if (!D->d_sync) {
+    if (dev_malloc((void**)&D->d_sync, sizeof(int) * (size_t)(D->sync_n + 1)) != 0) return -1;
+    HIPCHECK(hipMemsetAsync(D->d_sync, 0, sizeof(int) * (size_t)(D->sync_n + 1), st));
   }
   if (!g_graphs) {
     Emitter E = eager_emitter(D);
@@ -1296,6 +1439,21 @@ extern "C" int tamd_dev_stream_of(tamd_factor* F, hipStream_t* st)
   return 0;
 }
 
+// A chained sweep whose wait timed out leaves a mark in the error word (its
+// results are not valid); read and cleared after the caller's synchronization.
+extern "C" int tamd_dev_sweep_error(tamd_factor* F)
+{
+  tamd_dev* D = F ? F->dev : nullptr;
+  if (!D || !D->d_sync) return 0;
+  int e = 0;
+  HIPCHECK(hipMemcpy(&e, D->d_sync + D->sync_n, sizeof(int), hipMemcpyDeviceToHost));
+  if (!e) return 0;
+  HIPCHECK(hipMemsetAsync(D->d_sync + D->sync_n, 0, sizeof(int), D->stream));
+  HIPCHECK(hipStreamSynchronize(D->stream));
+  taucs_printf((char*)"taucs_amd: a solve sweep timed out waiting for a block\n");
+  return -1;
+}
+
 extern "C" int tamd_dev_solve_device(tamd_factor* F, int nrhs, double* dX, int ldx, const double* dB,
                                      int ldb)
 {
@@ -1310,7 +1468,7 @@ extern "C" int taucs_amd_supernodal_solve_many_device(void* vL, int nrhs, double
   int rc = solve_device(F, nrhs, dX, ldx, dB, ldb);
   if (rc) return rc;
   HIPCHECK(hipStreamSynchronize(F->dev->stream));
-  return 0;
+  return tamd_dev_sweep_error(F);
 }
 
 extern "C" int tamd_dev_solve(tamd_factor* F, int nrhs, double* X, int ldx, const double* B, int ldb)
@@ -1333,7 +1491,7 @@ extern "C" int tamd_dev_solve(tamd_factor* F, int nrhs, double* X, int ldx, cons
   HIPCHECK(hipMemcpy2DAsync(X, sizeof(double) * ldx, D->d_x, sizeof(double) * n, sizeof(double) * n,
                             nrhs, hipMemcpyDeviceToHost, D->stream));
   HIPCHECK(hipStreamSynchronize(D->stream));
-  return 0;
+  return tamd_dev_sweep_error(F);
 }
 
 extern "C" int tamd_dev_download_blocks(tamd_factor* F, double* Lhost)

@@ -409,26 +409,7 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
 // columns of one row: the epilogue's read-modify-write of C walks down
 // columns (16 consecutive rows per lane group).
 //
-// Lane layout of v_mfma_*_16x16x4 (checked on gfx950): A operand lane l holds
-// A[l%16][l/16], B operand lane l holds B[l/16][l%16].  Result register r of
-// lane l is D[l/16 + 4r][l%16] for f64 and D[4(l/16) + r][l%16] for f32.
-template <typename T> struct Mfma;
-template <> struct Mfma<double> {
-  typedef d4_t acc_t;
-  static __device__ acc_t step(double a, double b, acc_t c)
-  {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
-};
-template <> struct Mfma<float> {
-  typedef f4_t acc_t;
-  static __device__ acc_t step(float a, float b, acc_t c)
-  {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
+// (Mfma<T>: the 16x16x4 MFMA step and its lane layout, in kernels.h)
 
 #define TK 32
 #define TPAD 4

@@ -64,3 +64,24 @@ struct EATask {     // extend-add columns j0..j1-1 of a child's update block
 #define TAMD_OP_GEMM     5
 #define TAMD_OP_EXTEND   6
 #define TAMD_OP_ZERO     7
+
+// Lane layout of v_mfma_*_16x16x4 (checked on gfx950): A operand lane l holds
+// A[l%16][l/16], B operand lane l holds B[l/16][l%16].  Result register r of
+// lane l is D[l/16 + 4r][l%16] for f64 and D[4(l/16) + r][l%16] for f32.
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef d4_t acc_t;
+  static __device__ acc_t step(double a, double b, acc_t c)
+  {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+  typedef f4_t acc_t;
+  static __device__ acc_t step(float a, float b, acc_t c)
+  {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+};

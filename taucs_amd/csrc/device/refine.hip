@@ -310,6 +310,7 @@ static void colreduce(hipStream_t st, RefineCtx* c, const double* a, const doubl
 
 extern "C" int tamd_dev_stream_of(tamd_factor* F, hipStream_t* st);
 extern "C" int tamd_dev_solve_device(tamd_factor* F, int nrhs, double* dX, int ldx, const double* dB, int ldb);
+extern "C" int tamd_dev_sweep_error(tamd_factor* F);
 
 // Backward errors of x: r = b - A x,  eta = |r|_inf / (|A|_inf |x|_inf + |b|_inf)
 static int residual_and_error(hipStream_t st, RefineCtx* c, const double* x, const double* b,
@@ -432,6 +433,7 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
   HIPOK(hipMemcpy2DAsync(X, sizeof(double) * ldx, c->x, sizeof(double) * n, sizeof(double) * n, nrhs,
                          hipMemcpyDeviceToHost, st));
   HIPOK(hipStreamSynchronize(st));
+  if (tamd_dev_sweep_error(F)) return -1;
   if (info) {
     info->iterations = it;
     info->converged = conv;

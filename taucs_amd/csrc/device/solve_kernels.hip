@@ -22,19 +22,25 @@ namespace tamd {
 
 #define SMAXR 16
 
-// W_f <- [b(cols, :); 0]
+// W_f <- [b(cols, :); 0].  One wave per task (a front's rows r0 .. r0+nr-1,
+// nr <= 64), four tasks per workgroup.
 template <typename T>
-__global__ void k_solve_init(const FrontDesc* __restrict__ fr, const int* __restrict__ rows,
-                             const double* __restrict__ b, int ldb, T* __restrict__ W, int nrhs)
+__global__ __launch_bounds__(256) void k_solve_init(const FrontDesc* __restrict__ fr,
+                                                    const PanelTask* __restrict__ tasks, int count,
+                                                    const int* __restrict__ rows,
+                                                    const double* __restrict__ b, int ldb,
+                                                    T* __restrict__ W, int nrhs)
 {
-  const FrontDesc d = fr[blockIdx.x];
-  const int* rw = rows + d.rows;
-  const int tot = d.m * nrhs;
-  T* w = W + d.offW * nrhs;
-  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-    int i = e % d.m, r = e / d.m;
-    w[e] = i < d.s ? (T)b[rw[i] + (long long)r * ldb] : (T)0;
-  }
+  const int ti = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (ti >= count) return;
+  const PanelTask t = tasks[ti];
+  const FrontDesc d = fr[t.front];
+  const int i = t.r0 + lane;
+  if (lane >= t.nr) return;
+  T* w = W + d.offW * nrhs + i;
+  const int col = i < d.s ? rows[d.rows + i] : 0;
+  for (int r = 0; r < nrhs; r++)
+    w[(long long)r * d.m] = i < d.s ? (T)b[col + (long long)r * ldb] : (T)0;
 }
 
 // ---- LDS-tiled pieces ---------------------------------------------------------
@@ -353,9 +359,444 @@ __global__ __launch_bounds__(256) void k_bwd_step(const FrontDesc* __restrict__ 
   }
 }
 
+// ---- chained sweeps: one launch per level --------------------------------------
+// The block steps of a front's triangular solve form a chain (block i needs
+// the solution of blocks 0..i-1 forward, i+1..nb-1 backward).  Instead of one
+// launch per block step, one launch per level runs a task per 64-row tile of
+// every front, and a task reads the blocks it needs as soon as they exist.
+//
+// Readiness travels with the data: before a sweep, the forward solution Y
+// and the solution x are filled with a signaling-NaN bit pattern (no
+// arithmetic result is a signaling NaN), every value a task publishes is one
+// relaxed device-scope atomic store, and a reader polls each value it needs
+// with relaxed device-scope atomic loads until the pattern is gone.  Each
+// value is its own flag, so no fence (and no L2 write-back or invalidation)
+// is needed on either side.
+//
+// Forward progress does not depend on how many workgroups are resident: a
+// task takes its index from a ticket counter, tasks are listed so that every
+// task waits only for tasks of smaller index, and a task that holds a ticket
+// is running, so the smallest unfinished ticket can always proceed.  Every
+// wait is still bounded: after SPIN_LIMIT polls the value is taken as it is,
+// the error word is marked and the task finishes (the solve reports failure).
+// Launches whose tasks wait for nothing run with ticket == nullptr (task =
+// blockIdx).
+
+#define SPIN_LIMIT (1 << 20)
+
+template <typename T> struct Sentinel;
+template <> struct Sentinel<double> {
+  typedef unsigned long long U;
+  static constexpr U bits = 0x7FF0DEAD0000BEEFull;
+};
+template <> struct Sentinel<float> {
+  typedef unsigned int U;
+  static constexpr U bits = 0x7F80BEEFu;
+};
+
+template <typename T>
+__device__ __forceinline__ void publish(T* p, T v)
+{
+  typedef typename Sentinel<T>::U U;
+  __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Right-hand-side blocks of the chained kernels live in LDS as 64 x 16
+// arrays [row][rhs] with row stride YS (padded against bank conflicts).
+#define YS 17
+
+// Vl[i*YS + r] <- V[cols[i] + r*ldv] for i < nv, r < nrhs (0 elsewhere in the
+// 64 x 16 block), awaiting each value; all of a thread's loads are issued
+// before it waits for any.
+template <typename T, typename V>
+__device__ __forceinline__ void await_rows(T* Vl, const V* Vg, int ldv, const int* cols, int nv,
+                                           int nrhs, int* err)
+{
+  typedef typename Sentinel<V>::U U;
+  U bits[4];
+  U* adr[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int e = threadIdx.x + 256 * k, i = e & 63, r = e >> 6;
+    adr[k] = nullptr;
+    bits[k] = 0;
+    if (r < nrhs && i < nv) {
+      adr[k] = reinterpret_cast<U*>(const_cast<V*>(Vg + cols[i] + (long long)r * ldv));
+      bits[k] = __hip_atomic_load(adr[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int e = threadIdx.x + 256 * k, i = e & 63, r = e >> 6;
+    if (adr[k]) {
+      int it = 0;
+      while (bits[k] == Sentinel<V>::bits) {
+        if (++it > SPIN_LIMIT) {
+          atomicOr(err, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        bits[k] = __hip_atomic_load(adr[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    Vl[i * YS + r] = adr[k] ? (T)__builtin_bit_cast(V, bits[k]) : (T)0;
+  }
+}
+
+// Y (n x nrhs, ld n) and x (rows < n of n x nrhs, ld ldx) <- the pattern;
+// tickets <- 0.  The first op of every chained solve.
+template <typename T>
+__global__ void k_sweep_clear(T* __restrict__ Y, double* __restrict__ x, int n, int ldx, int nrhs,
+                              int* __restrict__ tickets, int nt)
+{
+  const long long tot = (long long)n * nrhs;
+  const T ys = __builtin_bit_cast(T, Sentinel<T>::bits);
+  const double xs = __builtin_bit_cast(double, Sentinel<double>::bits);
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long r = e / n, i = e - r * n;
+    Y[e] = ys;
+    x[i + r * ldx] = xs;
+  }
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nt; e += gridDim.x * blockDim.x)
+    tickets[e] = 0;
+}
+
+// Forward sweep of one level (fronts with m > 64).  Task (front, tile t):
+// rows [64t, 64t+64) of the front's workspace W_f (m x nrhs, the right-hand
+// side plus the children's updates).  Tile t < nb (nb = ceil(s/64)) owns
+// diagonal block t: it subtracts L[rows, blk j] y_j for j < t as the y_j
+// appear, forms y_t = inv(L_tt) w_t, publishes it in Y (by global column),
+// and (last block only) updates the sub-diagonal rows that share its tile.
+// Tiles t >= nb hold sub-diagonal rows only: they subtract all nb blocks and
+// store the result in W_f for the parent's extend-add.
+// The products run on the 16x16x4 MFMA: wave w owns rows 16w..16w+15 of the
+// tile and right-hand sides 0..15 (those beyond nrhs are zero), A = L rows
+// (loaded one dependency ahead), B = the awaited y_j block in LDS; the sum
+// over dependencies stays in the accumulators.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_chain(const FrontDesc* __restrict__ fr,
+                                                   const PanelTask* __restrict__ tasks,
+                                                   const int* __restrict__ rows,
+                                                   const T* __restrict__ L,
+                                                   const T* __restrict__ Inv, T* __restrict__ W,
+                                                   T* Y, int ldy, int* ticket, int* err, int nrhs)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T It[64 * 65];
+  __shared__ T Yl[2][64 * YS];
+  __shared__ int sh_task;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, c16 = lane & 15, kq = lane >> 4;
+  if (tid == 0) sh_task = ticket ? atomicAdd(ticket, 1) : (int)blockIdx.x;
+  __syncthreads();
+  const PanelTask t = tasks[sh_task];
+  const FrontDesc d = fr[t.front];
+  const int m = d.m, s = d.s, nb = (s + 63) >> 6, tile = t.k0, r0 = tile * 64;
+  const int nr = min(64, m - r0);
+  const bool isblk = tile < nb;
+  const int kb = isblk ? min(64, s - r0) : 0;
+  const int ndep = isblk ? tile : nb;
+  T* w = W + d.offW * nrhs;
+  const T* Lf = L + d.offL;
+  const int* rw = rows + d.rows;  // global column of each of the front's first s rows
+  const int arow = 16 * wv + c16;
+  T w0[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int i = 16 * wv + Mfma<T>::drow(lane, k);
+    w0[k] = (c16 < nrhs && i < nr) ? w[r0 + i + (long long)c16 * m] : (T)0;
+  }
+  if (isblk) {
+    const T* iv = Inv + d.offI + (long long)tile * 4096;
+    for (int e = tid; e < 4096; e += 256) It[(e >> 6) + (e & 63) * 65] = iv[e];
+  }
+  acc_t D;
+#pragma unroll
+  for (int k = 0; k < 4; k++) D[k] = (T)0;
+  T lv[16];
+  auto fetch = [&](int j) {
+    const int c0 = j * 64, kbj = min(64, s - c0);
+#pragma unroll
+    for (int st = 0; st < 16; st++) {
+      const int k = 4 * st + kq;
+      lv[st] = (arow < nr && k < kbj) ? Lf[r0 + arow + (long long)(c0 + k) * m] : (T)0;
+    }
+  };
+  if (ndep > 0) fetch(0);
+  for (int j = 0; j < ndep; j++) {
+    T* yb = Yl[j & 1];
+    await_rows(yb, Y, ldy, rw + 64 * j, min(64, s - 64 * j), nrhs, err);
+    __syncthreads();
+    T cur[16];
+#pragma unroll
+    for (int st = 0; st < 16; st++) cur[st] = lv[st];
+    if (j + 1 < ndep) fetch(j + 1);
+#pragma unroll
+    for (int st = 0; st < 16; st++) D = Mfma<T>::step(cur[st], yb[(4 * st + kq) * YS + c16], D);
+  }
+  T acc[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) acc[k] = w0[k] - D[k];
+  if (!isblk) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int i = 16 * wv + Mfma<T>::drow(lane, k);
+      if (c16 < nrhs && i < nr) w[r0 + i + (long long)c16 * m] = acc[k];
+    }
+    return;
+  }
+  // y = inv(L_tt) acc: A = inverse rows (zero beyond kb), B = acc (64 x 16)
+  T* Ac = Yl[ndep & 1];  // not read since the last dependency's barrier
+#pragma unroll
+  for (int k = 0; k < 4; k++) Ac[(16 * wv + Mfma<T>::drow(lane, k)) * YS + c16] = acc[k];
+  __syncthreads();
+  acc_t Yd;
+#pragma unroll
+  for (int k = 0; k < 4; k++) Yd[k] = (T)0;
+#pragma unroll
+  for (int st = 0; st < 16; st++)
+    Yd = Mfma<T>::step(It[arow + (4 * st + kq) * 65], Ac[(4 * st + kq) * YS + c16], Yd);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int i = 16 * wv + Mfma<T>::drow(lane, k);
+    if (i < kb && c16 < nrhs) publish(Y + rw[r0 + i] + (long long)c16 * ldy, (T)Yd[k]);
+  }
+  if (nr > kb) {  // sub-diagonal rows in the last block's tile: acc -= L[rows, blk t] y_t
+    T* Yv = Yl[(ndep + 1) & 1];  // free since the barrier above
+#pragma unroll
+    for (int k = 0; k < 4; k++) Yv[(16 * wv + Mfma<T>::drow(lane, k)) * YS + c16] = Yd[k];
+    __syncthreads();
+    acc_t D2;
+#pragma unroll
+    for (int k = 0; k < 4; k++) D2[k] = (T)0;
+#pragma unroll
+    for (int st = 0; st < 16; st++) {
+      const int k = 4 * st + kq;
+      const T a = (arow < nr && k < kb) ? Lf[r0 + arow + (long long)(r0 + k) * m] : (T)0;
+      D2 = Mfma<T>::step(a, Yv[k * YS + c16], D2);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int i = 16 * wv + Mfma<T>::drow(lane, k);
+      if (i >= kb && i < nr && c16 < nrhs) w[r0 + i + (long long)c16 * m] = acc[k] - D2[k];
+    }
+  }
+}
+
+// Backward sweep of one level (fronts with m > 64; the update rows' part
+// y -= L21^T x2 is already subtracted in Y by the split-K kernels).  Task
+// (front, block b): z = y_b - sum_{j > b} L[blk j, blk b]^T x_j with the x_j
+// of the same front awaited as they are published; then x_b = inv(L_bb)^T z
+// is published in x.  MFMA products as in k_fwd_chain with wave w owning the
+// block's columns 16w..16w+15: A = L^T (read down the block's columns, one
+// dependency ahead), B = the awaited x_j block in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_chain(const FrontDesc* __restrict__ fr,
+                                                   const PanelTask* __restrict__ tasks,
+                                                   const int* __restrict__ rows,
+                                                   const T* __restrict__ L,
+                                                   const T* __restrict__ Inv,
+                                                   const T* __restrict__ Y, int ldy, double* x,
+                                                   int ldx, int* ticket, int* err, int nrhs)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T It[64 * 65];
+  __shared__ T Xl[2][64 * YS];
+  __shared__ int sh_task;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, c16 = lane & 15, kq = lane >> 4;
+  if (tid == 0) sh_task = ticket ? atomicAdd(ticket, 1) : (int)blockIdx.x;
+  __syncthreads();
+  const PanelTask t = tasks[sh_task];
+  const FrontDesc d = fr[t.front];
+  const int m = d.m, s = d.s, nb = (s + 63) >> 6, blk = t.k0, c0 = blk * 64;
+  const int kb = min(64, s - c0);
+  const T* Lf = L + d.offL;
+  const int* rw = rows + d.rows;
+  const int acol = 16 * wv + c16;  // A-operand row = block column of this lane
+  T y0[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int q = 16 * wv + Mfma<T>::drow(lane, k);
+    y0[k] = (c16 < nrhs && q < kb) ? Y[rw[c0 + q] + (long long)c16 * ldy] : (T)0;
+  }
+  {
+    const T* iv = Inv + d.offI + (long long)blk * 4096;
+    for (int e = tid; e < 4096; e += 256) It[(e >> 6) + (e & 63) * 65] = iv[e];
+  }
+  const int ndep = nb - 1 - blk;  // blocks nb-1 down to blk+1
+  acc_t D;
+#pragma unroll
+  for (int k = 0; k < 4; k++) D[k] = (T)0;
+  T lv[16];
+  auto fetch = [&](int u) {
+    const int rr0 = 64 * (nb - 1 - u), nrr = min(64, s - rr0);
+#pragma unroll
+    for (int st = 0; st < 16; st++) {
+      const int i = 4 * st + kq;
+      lv[st] = (acol < kb && i < nrr) ? Lf[rr0 + i + (long long)(c0 + acol) * m] : (T)0;
+    }
+  };
+  if (ndep > 0) fetch(0);
+  for (int u = 0; u < ndep; u++) {
+    const int rr0 = 64 * (nb - 1 - u);
+    T* xb = Xl[u & 1];
+    await_rows(xb, x, ldx, rw + rr0, min(64, s - rr0), nrhs, err);
+    __syncthreads();
+    T cur[16];
+#pragma unroll
+    for (int st = 0; st < 16; st++) cur[st] = lv[st];
+    if (u + 1 < ndep) fetch(u + 1);
+#pragma unroll
+    for (int st = 0; st < 16; st++) D = Mfma<T>::step(cur[st], xb[(4 * st + kq) * YS + c16], D);
+  }
+  T* Z = Xl[ndep & 1];  // not read since the last dependency's barrier
+#pragma unroll
+  for (int k = 0; k < 4; k++) Z[(16 * wv + Mfma<T>::drow(lane, k)) * YS + c16] = y0[k] - D[k];
+  __syncthreads();
+  // x = inv(L_bb)^T z: A[q][p] = inv(p, q)
+  acc_t X;
+#pragma unroll
+  for (int k = 0; k < 4; k++) X[k] = (T)0;
+#pragma unroll
+  for (int st = 0; st < 16; st++)
+    X = Mfma<T>::step(It[(4 * st + kq) + acol * 65], Z[(4 * st + kq) * YS + c16], X);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int q = 16 * wv + Mfma<T>::drow(lane, k);
+    if (q < kb && c16 < nrhs) publish(x + rw[c0 + q] + (long long)c16 * ldx, (double)X[k]);
+  }
+}
+
+// Fronts with m <= 64, one wave each (four per workgroup), no waits: the
+// whole front's right-hand side fits one wave's rows.
+//   forward:  y = inv(L11) w[0:s] into Y; w[s:m] -= L21 y for the parent.
+//   backward: z = y - L21^T x(update rows); x = inv(L11)^T z.
+// Per-wave LDS holds the m x nrhs values that every lane needs.
+__device__ __forceinline__ void wave_sync_lds()
+{
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_small(const FrontDesc* __restrict__ fr,
+                                                   const PanelTask* __restrict__ tasks, int count,
+                                                   const int* __restrict__ rows,
+                                                   const T* __restrict__ L,
+                                                   const T* __restrict__ Inv, T* __restrict__ W,
+                                                   T* __restrict__ Y, int ldy, int nrhs)
+{
+  __shared__ T Ws[4][64 * SMAXR];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ti = blockIdx.x * 4 + wv;
+  if (ti >= count) return;
+  const FrontDesc d = fr[tasks[ti].front];
+  const int m = d.m, s = d.s;
+  T* w = W + d.offW * nrhs;
+  T* ws = Ws[wv];
+  const int* rw = rows + d.rows;
+  T v[SMAXR];
+#pragma unroll
+  for (int r = 0; r < SMAXR; r++)
+    if (r < nrhs) {
+      v[r] = lane < m ? w[lane + r * m] : (T)0;
+      ws[lane + r * 64] = v[r];
+    }
+  wave_sync_lds();
+  const T* iv = Inv + d.offI + lane * 64;
+  T y[SMAXR];
+#pragma unroll
+  for (int r = 0; r < SMAXR; r++) y[r] = (T)0;
+  for (int p = 0; p < s; p++) {
+    const T a = (lane < s && p <= lane) ? iv[p] : (T)0;
+#pragma unroll
+    for (int r = 0; r < SMAXR; r++)
+      if (r < nrhs) y[r] += a * ws[p + r * 64];
+  }
+  wave_sync_lds();
+  if (lane < s) {
+#pragma unroll
+    for (int r = 0; r < SMAXR; r++)
+      if (r < nrhs) {
+        ws[lane + r * 64] = y[r];
+        Y[rw[lane] + (long long)r * ldy] = y[r];
+      }
+  }
+  wave_sync_lds();
+  if (m == s) return;
+  const T* Lf = L + d.offL;
+  for (int p = 0; p < s; p++) {
+    const T a = (lane >= s && lane < m) ? Lf[lane + (long long)p * m] : (T)0;
+#pragma unroll
+    for (int r = 0; r < SMAXR; r++)
+      if (r < nrhs) v[r] -= a * ws[p + r * 64];
+  }
+  if (lane >= s && lane < m) {
+#pragma unroll
+    for (int r = 0; r < SMAXR; r++)
+      if (r < nrhs) w[lane + r * m] = v[r];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_small(const FrontDesc* __restrict__ fr,
+                                                   const PanelTask* __restrict__ tasks, int count,
+                                                   const int* __restrict__ rows,
+                                                   const T* __restrict__ L,
+                                                   const T* __restrict__ Inv,
+                                                   const T* __restrict__ Y, int ldy,
+                                                   double* __restrict__ x, int ldx, int nrhs)
+{
+  __shared__ T Ws[4][64 * SMAXR];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ti = blockIdx.x * 4 + wv;
+  if (ti >= count) return;
+  const FrontDesc d = fr[tasks[ti].front];
+  const int m = d.m, s = d.s;
+  const int* rw = rows + d.rows;
+  T* ws = Ws[wv];
+  T z[SMAXR];
+#pragma unroll
+  for (int r = 0; r < SMAXR; r++)
+    if (r < nrhs) {
+      z[r] = lane < s ? Y[rw[lane] + (long long)r * ldy] : (T)0;
+      ws[lane + r * 64] = (lane >= s && lane < m) ? (T)x[rw[lane] + (long long)r * ldx] : (T)0;
+    }
+  wave_sync_lds();
+  const T* Lf = L + d.offL + (long long)lane * m;  // column `lane` of the panel
+  for (int i = s; i < m; i++) {
+    const T a = lane < s ? Lf[i] : (T)0;
+#pragma unroll
+    for (int r = 0; r < SMAXR; r++)
+      if (r < nrhs) z[r] -= a * ws[i + r * 64];
+  }
+  if (lane < s) {
+#pragma unroll
+    for (int r = 0; r < SMAXR; r++)
+      if (r < nrhs) ws[lane + r * 64] = z[r];
+  }
+  wave_sync_lds();
+  const T* iv = Inv + d.offI + lane;  // column `lane` of the inverse: iv[p * 64]
+  T xv[SMAXR];
+#pragma unroll
+  for (int r = 0; r < SMAXR; r++) xv[r] = (T)0;
+  for (int p = 0; p < s; p++) {
+    const T a = (lane < s && p >= lane) ? iv[p * 64] : (T)0;
+#pragma unroll
+    for (int r = 0; r < SMAXR; r++)
+      if (r < nrhs) xv[r] += a * ws[p + r * 64];
+  }
+  if (lane < s) {
+#pragma unroll
+    for (int r = 0; r < SMAXR; r++)
+      if (r < nrhs) x[rw[lane] + (long long)r * ldx] = (double)xv[r];
+  }
+}
+
 #define TAMD_SINST(T)                                                                           \
-  template __global__ void k_solve_init<T>(const FrontDesc*, const int*, const double*, int, T*, \
-                                           int);                                                \
+  template __global__ void k_solve_init<T>(const FrontDesc*, const PanelTask*, int, const int*,  \
+                                           const double*, int, T*, int);                        \
   template __global__ void k_fwd_step<T>(const FrontDesc*, const PanelTask*, const int*,         \
                                          const T*, const T*, T*, T*, int, int);                 \
   template __global__ void k_fwd_rect<T>(const FrontDesc*, const PanelTask*, const int*,         \
@@ -369,7 +810,17 @@ __global__ __launch_bounds__(256) void k_bwd_step(const FrontDesc* __restrict__ 
   template __global__ void k_bwd_cols<T>(const FrontDesc*, const PanelTask*, const T*, const T*, \
                                          T*, int);                                              \
   template __global__ void k_bwd_step<T>(const FrontDesc*, const PanelTask*, const int*,         \
-                                         const T*, const T*, T*, int, double*, int, int);
+                                         const T*, const T*, T*, int, double*, int, int);       \
+  template __global__ void k_sweep_clear<T>(T*, double*, int, int, int, int*, int);           \
+  template __global__ void k_fwd_chain<T>(const FrontDesc*, const PanelTask*, const int*,        \
+                                          const T*, const T*, T*, T*, int, int*, int*, int);    \
+  template __global__ void k_bwd_chain<T>(const FrontDesc*, const PanelTask*, const int*,        \
+                                          const T*, const T*, const T*, int, double*, int,      \
+                                          int*, int*, int);                                     \
+  template __global__ void k_fwd_small<T>(const FrontDesc*, const PanelTask*, int, const int*,   \
+                                          const T*, const T*, T*, T*, int, int);                \
+  template __global__ void k_bwd_small<T>(const FrontDesc*, const PanelTask*, int, const int*,   \
+                                          const T*, const T*, const T*, int, double*, int, int);
 TAMD_SINST(double)
 TAMD_SINST(float)
 
