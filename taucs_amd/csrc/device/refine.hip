@@ -371,8 +371,16 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
   // Every right-hand side follows its own recurrence and stops changing once
   // its backward error reaches tol (its step is masked to zero), so a column's
   // result does not depend on which other columns are solved with it.
-  std::vector<int> done(nrhs, 0);
-  std::vector<double> mask(nrhs, 1.0);
+  std::vector<int> done(nrhs, 0), bad(nrhs, 0), stop(nrhs, 0);
+  std::vector<double> mask(nrhs, 1.0), best(nrhs, 1e300);
+  // A column diverges when its residual is not finite or has grown 1000x
+  // above the smallest seen: a diverging iterate can grow until |A||x|
+  // dwarfs the residual, and its backward error then looks small.
+  auto diverging = [&](int q) {
+    if (!(relres[q] <= 1e300) || !(eta[q] <= 1.0)) return true;
+    best[q] = std::min(best[q], relres[q]);
+    return relres[q] > 1e3 * best[q];
+  };
   if (method == 0) {
     // x = M^-1 b; repeat r = b - A x, x += M^-1 r
     if (tamd_dev_solve_device(F, nrhs, c->x, n, c->b, n)) return -1;
@@ -380,11 +388,11 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
       if (residual_and_error(st, c, c->x, c->b, c->r, nrhs, eta, relres)) return -1;
       bool all = true, settled = true;
       for (int q = 0; q < nrhs; q++) {
-        if (eta[q] <= tol) done[q] = 1;
-        const bool diverged = !(eta[q] <= 1.0);  // NaN or inf (eta <= 1 otherwise)
-        mask[q] = (done[q] || diverged) ? 0.0 : 1.0;
+        if (!done[q] && !bad[q] && diverging(q)) bad[q] = 1;
+        if (!bad[q] && eta[q] <= tol) done[q] = 1;
+        mask[q] = (done[q] || bad[q]) ? 0.0 : 1.0;
         all = all && done[q];
-        settled = settled && (done[q] || diverged);
+        settled = settled && (done[q] || bad[q]);
       }
       if (all) { conv = true; break; }
       if (settled) break;
@@ -413,14 +421,17 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
       it++;
       // convergence on the true residual every iteration (k small)
       if (residual_and_error(st, c, c->x, c->b, c->q, nrhs, eta, relres)) return -1;
-      bool all = true;
+      bool all = true, settled = true;
       for (int q = 0; q < nrhs; q++) {
-        if (eta[q] <= tol) done[q] = 1;
+        if (!done[q] && !bad[q] && diverging(q)) bad[q] = 1;
+        if (!bad[q] && eta[q] <= tol) done[q] = 1;
         all = all && done[q];
+        settled = settled && (done[q] || bad[q]);
+        stop[q] = done[q] || bad[q];  // its recurrence stops (the device mask)
       }
       if (all) { conv = true; break; }
-      if (it >= maxit) break;
-      HIPOK(hipMemcpyAsync(c->done, done.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+      if (settled || it >= maxit) break;
+      HIPOK(hipMemcpyAsync(c->done, stop.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
       if (tamd_dev_solve_device(F, nrhs, c->z, n, c->r, n)) return -1;
       colreduce(st, c, c->r, c->z, 0, c->rz_new, nrhs);
       hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, st, c->rz_new, c->rz, (const int*)nullptr, c->scal,
