@@ -679,17 +679,15 @@ __global__ __launch_bounds__(256) void k_gemm_tile_bf16(const GemmTask* __restri
 
 // Extend-add: child update block entries (i >= j) added into the parent's
 // panel or update block at the relative positions (src/taucs_sn_llt.c:1200-1240).
+// The child's update rows are ordered by their parent positions, so
+// r[i] > r[j] for i > j: entry (i, j) goes to parent (r[i], r[j]), below the
+// diagonal, and one child column lands in one parent column.  Each wave
+// takes every fourth column of the task; its lanes walk the column.
 template <typename T>
-__global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict__ fr,
-                                                    const EATask* __restrict__ tasks,
-                                                    const int* __restrict__ rel,
-                                                    T* __restrict__ L, T* __restrict__ U)
+__device__ __forceinline__ void extend_add_cols(const FrontDesc* __restrict__ fr, const EATask& t,
+                                                const int* __restrict__ rel, T* __restrict__ L,
+                                                T* __restrict__ U)
 {
-  // The child's update rows are ordered by their parent positions, so
-  // r[i] > r[j] for i > j: entry (i, j) goes to parent (r[i], r[j]), below the
-  // diagonal, and one child column lands in one parent column.  Each wave
-  // takes every fourth column of the task; its lanes walk the column.
-  const EATask t = tasks[blockIdx.x];
   const FrontDesc c = fr[t.child];
   const FrontDesc p = fr[c.parent];
   const int uc = c.m - c.s, sp = p.s, mp = p.m, up = mp - sp;
@@ -707,6 +705,33 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict__ fr,
+                                                    const EATask* __restrict__ tasks,
+                                                    const int* __restrict__ rel,
+                                                    T* __restrict__ L, T* __restrict__ U)
+{
+  extend_add_cols(fr, tasks[blockIdx.x], rel, L, U);
+}
+
+// One parent per workgroup: heads[b] = {first, count} names the tasks (whole
+// children, in sibling-rank order) added one after the other, so each parent
+// entry receives its children's values in the same order as under the
+// per-rank launches.
+template <typename T>
+__global__ __launch_bounds__(256) void k_extend_add_seq(const FrontDesc* __restrict__ fr,
+                                                        const EATask* __restrict__ heads,
+                                                        const EATask* __restrict__ tasks,
+                                                        const int* __restrict__ rel,
+                                                        T* __restrict__ L, T* __restrict__ U)
+{
+  const EATask h = heads[blockIdx.x];
+  for (int q = 0; q < h.j0; q++) {
+    if (q > 0) __syncthreads();
+    extend_add_cols(fr, tasks[h.child + q], rel, L, U);
+  }
+}
+
 #define TAMD_INST(T)                                                                   \
   template __global__ void k_fill<T>(T*, long long);                                   \
   template __global__ void k_fill_ranges<T>(const ZeroTask*, T*);                       \
@@ -719,7 +744,9 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
   template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_trsm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_gemm_tile128<T>(const GemmTask*, const T*, T*, T*);          \
-  template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*);
+  template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*); \
+  template __global__ void k_extend_add_seq<T>(const FrontDesc*, const EATask*, const EATask*,   \
+                                               const int*, T*, T*);
 TAMD_INST(double)
 TAMD_INST(float)
 template __global__ void k_fill<int>(int*, long long);
