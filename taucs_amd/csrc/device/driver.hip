@@ -41,9 +41,10 @@ template <typename T> __global__ void k_bwd_cols(const FrontDesc*, const PanelTa
 template <typename T> __global__ void k_bwd_step(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, T*, int, double*, int, int);
 template <typename T> __global__ void k_sweep_clear(T*, double*, int, int, int, int*, int);
 template <typename T> __global__ void k_fwd_chain(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, T*, T*, int, int*, int*, int);
-template <typename T> __global__ void k_bwd_chain(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, const T*, int, double*, int, int*, int*, int);
-template <typename T> __global__ void k_fwd_small(const FrontDesc*, const PanelTask*, int, const int*, const T*, const T*, T*, T*, int, int);
-template <typename T> __global__ void k_bwd_small(const FrontDesc*, const PanelTask*, int, const int*, const T*, const T*, const T*, int, double*, int, int);
+template <typename T> __global__ void k_bwd_chain(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, const T*, int, const T*, double*, int, int*, int*, int);
+template <typename T> __global__ void k_bwd_upd(const FrontDesc*, const PanelTask*, const int*, const T*, const double*, int, T*, int);
+template <typename T, int NR> __global__ void k_fwd_small(const FrontDesc*, const PanelTask*, int, const int*, const T*, const T*, T*, T*, int, int);
+template <typename T, int NR> __global__ void k_bwd_small(const FrontDesc*, const PanelTask*, int, const int*, const T*, const T*, const T*, int, double*, int, int);
 }  // namespace tamd
 
 using namespace tamd;
@@ -73,7 +74,7 @@ struct Op {
 
 // solve op kinds
 enum { SV_INIT, SV_FSTEP, SV_FRECT, SV_FRED, SV_EXT, SV_GATHER, SV_BCOLS, SV_BRED, SV_BSTEP,
-       SV_CLEAR, SV_FCHAIN, SV_BCHAIN, SV_FSMALL, SV_BSMALL };
+       SV_CLEAR, SV_FCHAIN, SV_BCHAIN, SV_FSMALL, SV_BSMALL, SV_BUPD };
 
 struct tamd_dev {
   hipStream_t stream = nullptr;
@@ -635,7 +636,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
   auto push_tasks = [&](int kind, size_t t0, int l) {
     if (svt.size() > t0) D->sv_ops.push_back({kind, (int)t0, (int)(svt.size() - t0), l});
   };
-  const int SB = 64, CH = 512, CHB = 128;
+  const int SB = 64, CH = 512, CHB = 128;  // CHB: rows per k_bwd_upd task (its BU_ROWS)
   D->part_slots = 0;
   for (int l = 0; l < nlev; l++) {
     int b0 = lvl[l], b1 = lvl[l + 1];
@@ -774,32 +775,25 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
   for (int l = nlev - 1; l >= 0; l--) {
     const int b0 = lvl[l], b1 = lvl[l + 1];
     size_t t0 = svt.size();
-    // fronts with m > 64: y -= L21^T x2 first, split over row chunks
-    // (k_bwd_gather, k_bwd_cols, k_bwd_cols_reduce), so the chained launch
-    // only follows the diagonal block chain
-    for (int k = b0; k < b1; k++) {
-      const int u = fr[k].m - fr[k].s;
-      if (fr[k].m <= 64) continue;
-      for (int r0 = 0; r0 < u; r0 += 256) svt.push_back({k, 0, 0, r0, std::min(256, u - r0)});
-    }
-    push_chain(SV_GATHER, t0, l, false);
+    // fronts with m > 64: the update rows' part L21^T x2 first, split over
+    // row chunks (k_bwd_upd, one Part slot per chunk: block b of front k owns
+    // slots sl0[k] + b * scnt[k] ..); the chained launch sums its slots and
+    // follows the diagonal block chain
+    std::vector<int> sl0(nf, 0), scnt(nf, 0);
     {
       t0 = svt.size();
-      std::vector<PanelTask> red;
       for (int k = b0; k < b1; k++) {
         if (fr[k].m <= 64 || fr[k].m == fr[k].s) continue;
+        sl0[k] = (int)(svt.size() - t0);
+        scnt[k] = (fr[k].m - fr[k].s + CHB - 1) / CHB;
         for (int c0 = 0; c0 < fr[k].s; c0 += 64) {
-          int nc = std::min(64, fr[k].s - c0), slot0 = (int)(svt.size() - t0), cnt = 0;
-          for (int i0 = fr[k].s; i0 < fr[k].m; i0 += CHB, cnt++)
+          const int nc = std::min(64, fr[k].s - c0);
+          for (int i0 = fr[k].s; i0 < fr[k].m; i0 += CHB)
             svt.push_back({k, i0, std::min(CHB, fr[k].m - i0), c0, nc});
-          red.push_back({k, slot0, cnt, c0, nc});
         }
       }
       D->part_slots = std::max(D->part_slots, (long long)(svt.size() - t0));
-      push_chain(SV_BCOLS, t0, l, false);
-      t0 = svt.size();
-      svt.insert(svt.end(), red.begin(), red.end());
-      push_chain(SV_BRED, t0, l, false);
+      push_chain(SV_BUPD, t0, l, false);
     }
     t0 = svt.size();
     for (int k = b0; k < b1; k++)
@@ -807,7 +801,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     push_chain(SV_BSMALL, t0, l, false);
     t0 = svt.size();
     for (int k = b0; k < b1; k++)
-      if (fr[k].m > 64 && nblk(k) == 1) svt.push_back({k, 0, 0, 0, 0});
+      if (fr[k].m > 64 && nblk(k) == 1) svt.push_back({k, 0, sl0[k], 0, scnt[k]});
     push_chain(SV_BCHAIN, t0, l, false);
     t0 = svt.size();
     int maxb = 0;
@@ -815,7 +809,10 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       if (nblk(k) > 1) maxb = std::max(maxb, nblk(k));
     for (int q = 0; q < maxb; q++)
       for (int k = b0; k < b1; k++)
-        if (nblk(k) > 1 && q < nblk(k)) svt.push_back({k, nblk(k) - 1 - q, 0, 0, 0});
+        if (nblk(k) > 1 && q < nblk(k)) {
+          const int b = nblk(k) - 1 - q;
+          svt.push_back({k, b, sl0[k] + b * scnt[k], 0, scnt[k]});
+        }
     push_chain(SV_BCHAIN, t0, l, true);
   }
   D->sync_n = ntickets;
@@ -1389,17 +1386,30 @@ static int emit_solve(tamd_dev* D, Emitter& E, int n, int nrhs, double* dX, int 
              op.tile >= 0 ? tickets + op.tile : (int*)nullptr, err, nrhs);
         break;
       case SV_BCHAIN:
-        EMIT(0, true, k_bwd_chain<T>, grid, dim3(256), fr, tk, rows, L, Iv, (const T*)Y, n, dX, ldx,
-             op.tile >= 0 ? tickets + op.tile : (int*)nullptr, err, nrhs);
+        EMIT(0, true, k_bwd_chain<T>, grid, dim3(256), fr, tk, rows, L, Iv, (const T*)Y, n,
+             (const T*)Part, dX, ldx, op.tile >= 0 ? tickets + op.tile : (int*)nullptr, err, nrhs);
         break;
-      case SV_FSMALL:
-        EMIT(0, true, k_fwd_small<T>, dim3((op.count + 3) / 4), dim3(256), fr, tk, op.count, rows, L,
-             Iv, W, Y, n, nrhs);
+      case SV_FSMALL: {
+        auto k = nrhs <= 1   ? k_fwd_small<T, 1>
+                 : nrhs <= 4 ? k_fwd_small<T, 4>
+                 : nrhs <= 8 ? k_fwd_small<T, 8>
+                             : k_fwd_small<T, 16>;
+        EMIT(0, true, k, dim3((op.count + 3) / 4), dim3(256), fr, tk, op.count, rows, L, Iv, W, Y, n,
+             nrhs);
         break;
-      case SV_BSMALL:
-        EMIT(0, true, k_bwd_small<T>, dim3((op.count + 3) / 4), dim3(256), fr, tk, op.count, rows, L,
-             Iv, (const T*)Y, n, dX, ldx, nrhs);
+      }
+      case SV_BUPD:
+        EMIT(0, true, k_bwd_upd<T>, grid, dim3(256), fr, tk, rows, L, (const double*)dX, ldx, Part, nrhs);
         break;
+      case SV_BSMALL: {
+        auto k = nrhs <= 1   ? k_bwd_small<T, 1>
+                 : nrhs <= 4 ? k_bwd_small<T, 4>
+                 : nrhs <= 8 ? k_bwd_small<T, 8>
+                             : k_bwd_small<T, 16>;
+        EMIT(0, true, k, dim3((op.count + 3) / 4), dim3(256), fr, tk, op.count, rows, L, Iv,
+             (const T*)Y, n, dX, ldx, nrhs);
+        break;
+      }
     }
   }
   return 0;

@@ -584,9 +584,10 @@ __global__ __launch_bounds__(256) void k_fwd_chain(const FrontDesc* __restrict__
   }
 }
 
-// Backward sweep of one level (fronts with m > 64; the update rows' part
-// y -= L21^T x2 is already subtracted in Y by the split-K kernels).  Task
-// (front, block b): z = y_b - sum_{j > b} L[blk j, blk b]^T x_j with the x_j
+// Backward sweep of one level (fronts with m > 64).  Task (front, block b,
+// kb = first Part slot, nr = slot count): the update rows' part L21^T x2 of
+// the block's columns comes from k_bwd_upd as nr partial blocks, summed in
+// slot order; z = y_b - that - sum_{j > b} L[blk j, blk b]^T x_j with the x_j
 // of the same front awaited as they are published; then x_b = inv(L_bb)^T z
 // is published in x.  MFMA products as in k_fwd_chain with wave w owning the
 // block's columns 16w..16w+15: A = L^T (read down the block's columns, one
@@ -597,7 +598,8 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const FrontDesc* __restrict__
                                                    const int* __restrict__ rows,
                                                    const T* __restrict__ L,
                                                    const T* __restrict__ Inv,
-                                                   const T* __restrict__ Y, int ldy, double* x,
+                                                   const T* __restrict__ Y, int ldy,
+                                                   const T* __restrict__ Part, double* x,
                                                    int ldx, int* ticket, int* err, int nrhs)
 {
   typedef typename Mfma<T>::acc_t acc_t;
@@ -618,7 +620,18 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const FrontDesc* __restrict__
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int q = 16 * wv + Mfma<T>::drow(lane, k);
-    y0[k] = (c16 < nrhs && q < kb) ? Y[rw[c0 + q] + (long long)c16 * ldy] : (T)0;
+    const T* ps = Part + (long long)t.kb * (64 * SMAXR) + q + c16 * 64;
+    T s0 = (T)0, s1 = (T)0, s2 = (T)0, s3 = (T)0;  // slots in four interleaved sums
+    int u = 0;
+    for (; u + 3 < t.nr; u += 4) {
+      s0 += ps[(long long)u * (64 * SMAXR)];
+      s1 += ps[(long long)(u + 1) * (64 * SMAXR)];
+      s2 += ps[(long long)(u + 2) * (64 * SMAXR)];
+      s3 += ps[(long long)(u + 3) * (64 * SMAXR)];
+    }
+    for (; u < t.nr; u++) s0 += ps[(long long)u * (64 * SMAXR)];
+    const T sum = (s0 + s1) + (s2 + s3);
+    y0[k] = (c16 < nrhs && q < kb) ? Y[rw[c0 + q] + (long long)c16 * ldy] - sum : (T)0;
   }
   {
     const T* iv = Inv + d.offI + (long long)blk * 4096;
@@ -668,6 +681,55 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const FrontDesc* __restrict__
   }
 }
 
+// Backward sweep, update rows of the fronts with m > 64, split over row
+// chunks (before the chained launch): Part[slot] (64 columns x 16 rhs) =
+// L[rows i0 .. i0+cnt-1, columns c0 .. c0+nc-1]^T x(rows), task (front, k0 =
+// i0, kb = cnt <= BU_ROWS, r0 = c0, nr = nc); k_bwd_cols_reduce subtracts the
+// slots from Y in order.  x of the update rows is final (the ancestors'
+// levels wrote it) and is read through the front's row list.  MFMA as in
+// k_bwd_chain: wave w owns columns 16w..16w+15, A = L^T, B = x rows in LDS.
+#define BU_ROWS 128
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_upd(const FrontDesc* __restrict__ fr,
+                                                 const PanelTask* __restrict__ tasks,
+                                                 const int* __restrict__ rows,
+                                                 const T* __restrict__ L,
+                                                 const double* __restrict__ x, int ldx,
+                                                 T* __restrict__ Part, int nrhs)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T Xl[BU_ROWS * YS];
+  const PanelTask t = tasks[blockIdx.x];
+  const FrontDesc d = fr[t.front];
+  const int m = d.m, i0 = t.k0, cnt = t.kb, c0 = t.r0, nc = t.nr;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, c16 = lane & 15, kq = lane >> 4;
+  const int* rw = rows + d.rows;
+  const int acol = 16 * wv + c16;
+  const T* Lc = L + d.offL + i0 + (long long)(c0 + acol) * m;  // column c0+acol from row i0
+  T a[BU_ROWS / 4];
+#pragma unroll
+  for (int st = 0; st < BU_ROWS / 4; st++) {
+    const int i = 4 * st + kq;
+    a[st] = (acol < nc && i < cnt) ? Lc[i] : (T)0;
+  }
+#pragma unroll
+  for (int k = 0; k < BU_ROWS * 16 / 256; k++) {
+    const int e = tid + 256 * k, i = e & (BU_ROWS - 1), r = e / BU_ROWS;
+    Xl[i * YS + r] = (i < cnt && r < nrhs) ? (T)x[rw[i0 + i] + (long long)r * ldx] : (T)0;
+  }
+  __syncthreads();
+  acc_t D;
+#pragma unroll
+  for (int k = 0; k < 4; k++) D[k] = (T)0;
+  const int nst = (cnt + 3) >> 2;
+#pragma unroll
+  for (int st = 0; st < BU_ROWS / 4; st++)
+    if (st < nst) D = Mfma<T>::step(a[st], Xl[(4 * st + kq) * YS + c16], D);
+  T* part = Part + (long long)blockIdx.x * (64 * SMAXR);
+#pragma unroll
+  for (int k = 0; k < 4; k++) part[16 * wv + Mfma<T>::drow(lane, k) + c16 * 64] = D[k];
+}
+
 // Fronts with m <= 64, one wave each (four per workgroup), no waits: the
 // whole front's right-hand side fits one wave's rows.
 //   forward:  y = inv(L11) w[0:s] into Y; w[s:m] -= L21 y for the parent.
@@ -679,7 +741,9 @@ __device__ __forceinline__ void wave_sync_lds()
   __builtin_amdgcn_wave_barrier();
 }
 
-template <typename T>
+// NR: right-hand sides per pass rounded up to 1, 4, 8 or 16 (compile-time, so
+// the loops over them carry no guards); columns beyond nrhs hold zeros.
+template <typename T, int NR>
 __global__ __launch_bounds__(256) void k_fwd_small(const FrontDesc* __restrict__ fr,
                                                    const PanelTask* __restrict__ tasks, int count,
                                                    const int* __restrict__ rows,
@@ -687,7 +751,7 @@ __global__ __launch_bounds__(256) void k_fwd_small(const FrontDesc* __restrict__
                                                    const T* __restrict__ Inv, T* __restrict__ W,
                                                    T* __restrict__ Y, int ldy, int nrhs)
 {
-  __shared__ T Ws[4][64 * SMAXR];
+  __shared__ T Ws[4][64 * NR];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ti = blockIdx.x * 4 + wv;
   if (ti >= count) return;
@@ -696,32 +760,29 @@ __global__ __launch_bounds__(256) void k_fwd_small(const FrontDesc* __restrict__
   T* w = W + d.offW * nrhs;
   T* ws = Ws[wv];
   const int* rw = rows + d.rows;
-  T v[SMAXR];
+  T v[NR];
 #pragma unroll
-  for (int r = 0; r < SMAXR; r++)
-    if (r < nrhs) {
-      v[r] = lane < m ? w[lane + r * m] : (T)0;
-      ws[lane + r * 64] = v[r];
-    }
+  for (int r = 0; r < NR; r++) {
+    v[r] = (r < nrhs && lane < m) ? w[lane + r * m] : (T)0;
+    ws[lane + r * 64] = v[r];
+  }
   wave_sync_lds();
   const T* iv = Inv + d.offI + lane * 64;
-  T y[SMAXR];
+  T y[NR];
 #pragma unroll
-  for (int r = 0; r < SMAXR; r++) y[r] = (T)0;
+  for (int r = 0; r < NR; r++) y[r] = (T)0;
   for (int p = 0; p < s; p++) {
     const T a = (lane < s && p <= lane) ? iv[p] : (T)0;
 #pragma unroll
-    for (int r = 0; r < SMAXR; r++)
-      if (r < nrhs) y[r] += a * ws[p + r * 64];
+    for (int r = 0; r < NR; r++) y[r] += a * ws[p + r * 64];
   }
   wave_sync_lds();
   if (lane < s) {
 #pragma unroll
-    for (int r = 0; r < SMAXR; r++)
-      if (r < nrhs) {
-        ws[lane + r * 64] = y[r];
-        Y[rw[lane] + (long long)r * ldy] = y[r];
-      }
+    for (int r = 0; r < NR; r++) {
+      ws[lane + r * 64] = y[r];
+      if (r < nrhs) Y[rw[lane] + (long long)r * ldy] = y[r];
+    }
   }
   wave_sync_lds();
   if (m == s) return;
@@ -729,17 +790,16 @@ __global__ __launch_bounds__(256) void k_fwd_small(const FrontDesc* __restrict__
   for (int p = 0; p < s; p++) {
     const T a = (lane >= s && lane < m) ? Lf[lane + (long long)p * m] : (T)0;
 #pragma unroll
-    for (int r = 0; r < SMAXR; r++)
-      if (r < nrhs) v[r] -= a * ws[p + r * 64];
+    for (int r = 0; r < NR; r++) v[r] -= a * ws[p + r * 64];
   }
   if (lane >= s && lane < m) {
 #pragma unroll
-    for (int r = 0; r < SMAXR; r++)
+    for (int r = 0; r < NR; r++)
       if (r < nrhs) w[lane + r * m] = v[r];
   }
 }
 
-template <typename T>
+template <typename T, int NR>
 __global__ __launch_bounds__(256) void k_bwd_small(const FrontDesc* __restrict__ fr,
                                                    const PanelTask* __restrict__ tasks, int count,
                                                    const int* __restrict__ rows,
@@ -748,7 +808,7 @@ __global__ __launch_bounds__(256) void k_bwd_small(const FrontDesc* __restrict__
                                                    const T* __restrict__ Y, int ldy,
                                                    double* __restrict__ x, int ldx, int nrhs)
 {
-  __shared__ T Ws[4][64 * SMAXR];
+  __shared__ T Ws[4][64 * NR];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ti = blockIdx.x * 4 + wv;
   if (ti >= count) return;
@@ -756,44 +816,46 @@ __global__ __launch_bounds__(256) void k_bwd_small(const FrontDesc* __restrict__
   const int m = d.m, s = d.s;
   const int* rw = rows + d.rows;
   T* ws = Ws[wv];
-  T z[SMAXR];
+  T z[NR];
 #pragma unroll
-  for (int r = 0; r < SMAXR; r++)
-    if (r < nrhs) {
-      z[r] = lane < s ? Y[rw[lane] + (long long)r * ldy] : (T)0;
-      ws[lane + r * 64] = (lane >= s && lane < m) ? (T)x[rw[lane] + (long long)r * ldx] : (T)0;
-    }
+  for (int r = 0; r < NR; r++) {
+    z[r] = (r < nrhs && lane < s) ? Y[rw[lane] + (long long)r * ldy] : (T)0;
+    ws[lane + r * 64] =
+        (r < nrhs && lane >= s && lane < m) ? (T)x[rw[lane] + (long long)r * ldx] : (T)0;
+  }
   wave_sync_lds();
   const T* Lf = L + d.offL + (long long)lane * m;  // column `lane` of the panel
   for (int i = s; i < m; i++) {
     const T a = lane < s ? Lf[i] : (T)0;
 #pragma unroll
-    for (int r = 0; r < SMAXR; r++)
-      if (r < nrhs) z[r] -= a * ws[i + r * 64];
+    for (int r = 0; r < NR; r++) z[r] -= a * ws[i + r * 64];
   }
   if (lane < s) {
 #pragma unroll
-    for (int r = 0; r < SMAXR; r++)
-      if (r < nrhs) ws[lane + r * 64] = z[r];
+    for (int r = 0; r < NR; r++) ws[lane + r * 64] = z[r];
   }
   wave_sync_lds();
   const T* iv = Inv + d.offI + lane;  // column `lane` of the inverse: iv[p * 64]
-  T xv[SMAXR];
+  T xv[NR];
 #pragma unroll
-  for (int r = 0; r < SMAXR; r++) xv[r] = (T)0;
+  for (int r = 0; r < NR; r++) xv[r] = (T)0;
   for (int p = 0; p < s; p++) {
     const T a = (lane < s && p >= lane) ? iv[p * 64] : (T)0;
 #pragma unroll
-    for (int r = 0; r < SMAXR; r++)
-      if (r < nrhs) xv[r] += a * ws[p + r * 64];
+    for (int r = 0; r < NR; r++) xv[r] += a * ws[p + r * 64];
   }
   if (lane < s) {
 #pragma unroll
-    for (int r = 0; r < SMAXR; r++)
+    for (int r = 0; r < NR; r++)
       if (r < nrhs) x[rw[lane] + (long long)r * ldx] = (double)xv[r];
   }
 }
 
+#define SMALL_INST(T, NR)                                                                       \
+  template __global__ void k_fwd_small<T, NR>(const FrontDesc*, const PanelTask*, int, const int*, \
+                                              const T*, const T*, T*, T*, int, int);            \
+  template __global__ void k_bwd_small<T, NR>(const FrontDesc*, const PanelTask*, int, const int*, \
+                                              const T*, const T*, const T*, int, double*, int, int);
 #define TAMD_SINST(T)                                                                           \
   template __global__ void k_solve_init<T>(const FrontDesc*, const PanelTask*, int, const int*,  \
                                            const double*, int, T*, int);                        \
@@ -815,12 +877,11 @@ __global__ __launch_bounds__(256) void k_bwd_small(const FrontDesc* __restrict__
   template __global__ void k_fwd_chain<T>(const FrontDesc*, const PanelTask*, const int*,        \
                                           const T*, const T*, T*, T*, int, int*, int*, int);    \
   template __global__ void k_bwd_chain<T>(const FrontDesc*, const PanelTask*, const int*,        \
-                                          const T*, const T*, const T*, int, double*, int,      \
-                                          int*, int*, int);                                     \
-  template __global__ void k_fwd_small<T>(const FrontDesc*, const PanelTask*, int, const int*,   \
-                                          const T*, const T*, T*, T*, int, int);                \
-  template __global__ void k_bwd_small<T>(const FrontDesc*, const PanelTask*, int, const int*,   \
-                                          const T*, const T*, const T*, int, double*, int, int);
+                                          const T*, const T*, const T*, int, const T*, double*, \
+                                          int, int*, int*, int);                                \
+  template __global__ void k_bwd_upd<T>(const FrontDesc*, const PanelTask*, const int*, const T*, \
+                                        const double*, int, T*, int);                          \
+  SMALL_INST(T, 1) SMALL_INST(T, 4) SMALL_INST(T, 8) SMALL_INST(T, 16)
 TAMD_SINST(double)
 TAMD_SINST(float)
 
