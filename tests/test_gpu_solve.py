@@ -1,0 +1,100 @@
+"""GPU: the two sweep schedules of the supernodal solve, and refinement that
+must not report a diverging iterate as converged.
+
+The chained sweeps (one launch per tree level, block chains followed inside
+the launch) and the per-block-step sweeps compute the same triangular solves
+in different summation orders: their solutions agree to a few ulps of the
+condition number (rtol 1e-12 in FP64 on these small meshes, 1e-5 in FP32),
+and each against a dense solve.  Each schedule is deterministic: a repeated
+solve is bitwise equal."""
+import numpy as np
+import pytest
+
+import taucs_amd as T
+
+pytestmark = pytest.mark.gpu
+
+
+def factored(A, precision, chained):
+    perm, inv = T.order(A, "genmmd")
+    PA = T.permute(A, perm, inv)
+    T.lib.taucs_amd_set_solve_kernel(1 if chained else 0)
+    try:
+        F = T.factor_symbolic(PA)
+        assert F.numeric(PA, precision) == 0
+    finally:
+        T.lib.taucs_amd_set_solve_kernel(1)
+    return F, PA
+
+
+@pytest.mark.parametrize("precision,rtol", [(T.FP64, 1e-12), (T.FP32, 1e-5)])
+@pytest.mark.parametrize("nrhs", [1, 3, 8, 16, 20])
+def test_chained_and_step_sweeps_agree(precision, rtol, nrhs):
+    print(f"case: chained vs step sweeps, mesh3d 14, precision {precision}, nrhs {nrhs}", flush=True)
+    A = T.mesh3d(14)
+    Fc, PA = factored(A, precision, True)
+    Fs, _ = factored(A, precision, False)
+    rng = np.random.default_rng(nrhs)
+    B = rng.uniform(-1, 1, (PA.n, nrhs))
+    Xc = Fc.solve(B)
+    Xs = Fs.solve(B)
+    M = PA.to_scipy_full().toarray()
+    Xd = np.linalg.solve(M, B)
+    scale = np.abs(Xd).max()
+    assert np.abs(Xc - Xs).max() <= rtol * scale
+    if precision == T.FP64:
+        assert np.abs(Xc - Xd).max() <= 1e-10 * scale
+    else:  # FP32 factor and solve: normwise backward error of single precision
+        anorm = np.abs(M).sum(1).max()
+        eta = np.abs(B - M @ Xc).max(0) / (anorm * np.abs(Xc).max(0) + np.abs(B).max(0))
+        assert eta.max() <= 1e-5
+    assert np.array_equal(Fc.solve(B), Xc)
+    assert np.array_equal(Fs.solve(B), Xs)
+    Fc.free()
+    Fs.free()
+
+
+def test_chained_sweeps_on_a_deep_front_chain():
+    """Fronts wider than 64 columns at every top level: the root of mesh3d 24
+    has a chain of tens of 64-column blocks in both sweeps."""
+    print("case: chained sweeps, mesh3d 24, 4 rhs", flush=True)
+    A = T.mesh3d(24)
+    Fc, PA = factored(A, T.FP64, True)
+    Fs, _ = factored(A, T.FP64, False)
+    B = np.stack([T.random_uniform(PA.n, 40 + r) for r in range(4)], 1)
+    Xc = Fc.solve(B)
+    Xs = Fs.solve(B)
+    assert np.abs(Xc - Xs).max() <= 1e-11 * np.abs(Xs).max()
+    M = PA.to_scipy_full()
+    R = B - M @ Xc
+    assert (np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)).max() <= 1e-11
+    Fc.free()
+    Fs.free()
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_diverging_refinement_is_not_reported_converged(method):
+    """A preconditioner that is the factor of 0.3 A makes classical
+    refinement's error grow by 7/3 per step: the iterate's backward error can
+    still look small once |A||x| dwarfs the residual, so the growth of the
+    residual must stop it as diverged.  CG with the same (SPD, merely
+    mis-scaled) preconditioner converges."""
+    print(f"case: refinement with a mis-scaled factor, method {method}", flush=True)
+    A = T.mesh3d(10)
+    perm, inv = T.order(A, "genmmd")
+    PA = T.permute(A, perm, inv)
+    n, cp, ri, v = PA.arrays()
+    scaled = T.from_arrays(n, cp, ri, 0.3 * v)
+    F = T.factor_symbolic(scaled)
+    assert F.numeric(scaled, T.FP64) == 0
+    B = np.stack([T.random_uniform(n, 7 + r) for r in range(2)], 1)
+    X, info = F.solve_refined(PA, B, method=method, tol=1e-13, maxit=200)
+    if method == 0:
+        assert not info.converged
+        assert info.iterations < 200
+    else:
+        assert info.converged
+        assert info.backward_error <= 1e-13
+        M = PA.to_scipy_full()
+        assert np.abs(B - M @ X).max() <= 1e-10 * np.abs(B).max()
+    F.free()
