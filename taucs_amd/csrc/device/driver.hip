@@ -732,8 +732,9 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
   // last one), single-block fronts of the backward without tickets.  Op.tile
   // holds the launch's ticket index (-1: none).
   D->sv_chain_ops.clear();
-  D->sv_chain_ops.push_back({SV_CLEAR, 0, 0, -1});
+  // b is read into the workspaces before x is cleared: x may alias b
   D->sv_chain_ops.push_back({SV_INIT, 0, n_init, -1});
+  D->sv_chain_ops.push_back({SV_CLEAR, 0, 0, -1});
   int ntickets = 0;
   auto push_chain = [&](int kind, size_t t0, int l, bool ticketed) {
     if (svt.size() <= t0) return;

@@ -282,34 +282,80 @@ __device__ __forceinline__ void wave_chol_inv(T (&r)[64], int lane, int nsteps, 
   }
 }
 
+// X = L^{-1} of the 64 x 64 block by 16 x 16 blocks, one wave.  L goes to LDS
+// row-major (Ls[i*PW_LD + c]).  The four diagonal blocks are inverted by
+// forward substitution (lane = block b, column c; x_ic = (delta_ic - sum_t
+// l_it x_tc) / l_ii).  The off-diagonal blocks follow by block distance on
+// the 16x16x4 MFMA, X_ij = -X_ii (sum_{k=j}^{i-1} L_ik X_kj).  X is kept
+// transposed in the upper triangle of Ls (X(r, c) at Ls[c*PW_LD + r], r >= c;
+// L's diagonal is only needed through rdiag), so no second 64 x 64 buffer is
+// needed; reads of X mask the entries above its diagonal.  Registers then
+// hold column `lane` of the inverse, as the stores expect.  Rows with
+// rdiag = 1 and unit rows of Ls (the identity padding) give the identity.
 template <typename T>
-__device__ __forceinline__ void wave_inverse(T (&r)[64], int lane, int n, T* Ls, const T* rdiag)
+__device__ __forceinline__ void wave_inverse(T (&r)[64], int lane, T* Ls, const T* rdiag, T* Ts)
 {
-  // L rows to LDS; registers then hold column `lane` of the inverse
+  typedef typename Mfma<T>::acc_t acc_t;
 #pragma clang loop unroll(full)
   for (int c = 0; c < 64; c++) Ls[lane * PW_LD + c] = r[c];
   wave_lds_fence();
+  const int b = lane >> 4, c16 = lane & 15, kq = lane >> 4;
+  {
+    T x[16];
+    const T* Lb = Ls + (16 * b) * PW_LD + 16 * b;
 #pragma clang loop unroll(full)
-  for (int i = 0; i < 64; i++) {
-    if (i < n) {
-      const T* li = Ls + i * PW_LD;
-      T a0 = (i == lane) ? (T)1 : (T)0, a1 = (T)0;
+    for (int i = 0; i < 16; i++) {
+      T a0 = (i == c16) ? (T)1 : (T)0, a1 = (T)0;
 #pragma clang loop unroll(full)
-      for (int t0 = 0; t0 < i; t0 += 16) {
-        T v[16];
-#pragma clang loop unroll(full)
-        for (int q = 0; q < 16; q++)
-          if (t0 + q < i) v[q] = li[t0 + q];
-        __builtin_amdgcn_sched_barrier(0);
-#pragma clang loop unroll(full)
-        for (int q = 0; q < 16; q += 2) {
-          if (t0 + q < i) a0 -= v[q] * r[t0 + q];
-          if (t0 + q + 1 < i) a1 -= v[q + 1] * r[t0 + q + 1];
-        }
+      for (int t = 0; t < i; t += 2) {
+        a0 -= Lb[i * PW_LD + t] * x[t];
+        if (t + 1 < i) a1 -= Lb[i * PW_LD + t + 1] * x[t + 1];
       }
-      r[i] = (a0 + a1) * rdiag[i];
+      x[i] = (a0 + a1) * rdiag[16 * b + i];
+    }
+    // X(16b+i, 16b+c16) for i >= c16, at the transposed (upper) position
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 16; i++)
+      if (i >= c16) Ls[(16 * b + c16) * PW_LD + 16 * b + i] = x[i];
+  }
+  wave_lds_fence();
+#pragma clang loop unroll(full)
+  for (int d = 1; d < 4; d++) {
+#pragma clang loop unroll(full)
+    for (int j = 0; j + d < 4; j++) {
+      const int i = j + d;
+      acc_t acc;
+#pragma clang loop unroll(full)
+      for (int q = 0; q < 4; q++) acc[q] = (T)0;
+#pragma clang loop unroll(full)
+      for (int k = j; k < i; k++)
+#pragma clang loop unroll(full)
+        for (int st = 0; st < 4; st++) {
+          const int kr = 4 * st + kq;  // row of X_kj, column of L_ik
+          const T a = Ls[(16 * i + c16) * PW_LD + 16 * k + kr];
+          const T xv = (k > j || kr >= c16) ? Ls[(16 * j + c16) * PW_LD + 16 * k + kr] : (T)0;
+          acc = Mfma<T>::step(a, xv, acc);
+        }
+#pragma clang loop unroll(full)
+      for (int q = 0; q < 4; q++) Ts[Mfma<T>::drow(lane, q) * 17 + c16] = acc[q];
+      wave_lds_fence();
+      acc_t xr;
+#pragma clang loop unroll(full)
+      for (int q = 0; q < 4; q++) xr[q] = (T)0;
+#pragma clang loop unroll(full)
+      for (int st = 0; st < 4; st++) {
+        const int kr = 4 * st + kq;  // column of X_ii
+        const T a = (c16 >= kr) ? Ls[(16 * i + kr) * PW_LD + 16 * i + c16] : (T)0;
+        xr = Mfma<T>::step(a, Ts[kr * 17 + c16], xr);
+      }
+#pragma clang loop unroll(full)
+      for (int q = 0; q < 4; q++)
+        Ls[(16 * j + c16) * PW_LD + 16 * i + Mfma<T>::drow(lane, q)] = -xr[q];
+      wave_lds_fence();
     }
   }
+#pragma clang loop unroll(full)
+  for (int i = 0; i < 64; i++) r[i] = i >= lane ? Ls[lane * PW_LD + i] : (T)0;
   __builtin_amdgcn_wave_barrier();
 }
 
@@ -322,6 +368,7 @@ __global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__
   __shared__ __align__(16) T col[64];
   __shared__ __align__(16) T Ls[64 * PW_LD];
   __shared__ __align__(16) T rdiag[64];
+  __shared__ __align__(16) T Ts[16 * 17];
   const PanelTask t = tasks[blockIdx.x];
   const FrontDesc d = fr[t.front];
   const long long m = d.m;
@@ -338,7 +385,7 @@ __global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__
 #pragma unroll
   for (int c = 0; c < 64; c++)
     if (lane < kb && c < kb && lane >= c) gw[lane + c * m] = r[c];
-  wave_inverse<T>(r, lane, 64, Ls, rdiag);
+  wave_inverse<T>(r, lane, Ls, rdiag, Ts);
   T* iv = Inv + d.offI + (long long)(t.k0 / 64) * 4096;
   asm volatile("" : "+s"(iv));
 #pragma unroll
@@ -362,6 +409,7 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
   __shared__ __align__(16) T col[64];
   __shared__ __align__(16) T Ls[64 * PW_LD];
   __shared__ __align__(16) T rdiag[64];
+  __shared__ __align__(16) T Ts[16 * 17];
   const FrontDesc d = fr[list[blockIdx.x]];
   const int s = d.s, m = d.m, u = m - s, lane = threadIdx.x;
   T* gp = L + d.offL;
@@ -393,7 +441,14 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
       if (ln >= c && ln < m) guw[(ln - s) + (c - s) * u] = r[c];
     }
   }
-  wave_inverse<T>(r, ln, s, Ls, rdiag);
+  // the inverse of the leading s x s block: rows and columns beyond s become
+  // the identity (they hold the Schur complement, not L)
+#pragma unroll
+  for (int c = 0; c < 64; c++)
+    if (ln >= s) r[c] = c == ln ? (T)1 : (T)0;
+  if (ln >= s) rdiag[ln] = (T)1;
+  wave_lds_fence();
+  wave_inverse<T>(r, ln, Ls, rdiag, Ts);
   T* iv = Inv + d.offI;
   asm volatile("" : "+s"(iv));
   asm volatile("" : "+v"(ln));

@@ -98,3 +98,19 @@ def test_diverging_refinement_is_not_reported_converged(method):
         M = PA.to_scipy_full()
         assert np.abs(B - M @ X).max() <= 1e-10 * np.abs(B).max()
     F.free()
+
+
+def test_device_solve_in_place():
+    """x may alias b in the device-pointer solve: b is consumed before x is
+    written (the chained schedule's readiness fill of x comes after)."""
+    print("case: in-place device solve, mesh3d 16, 5 rhs", flush=True)
+    A = T.mesh3d(16)
+    F, PA = factored(A, T.FP64, True)
+    n = PA.n
+    B = np.stack([T.random_uniform(n, 90 + r) for r in range(5)], 1)
+    X = F.solve(B)
+    dB = T.DeviceArray(B)
+    F.solve_device(dB.ptr, dB.ptr, n, 5)
+    assert np.array_equal(dB.get(), X)
+    dB.free()
+    F.free()
