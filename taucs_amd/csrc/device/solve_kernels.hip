@@ -406,12 +406,13 @@ __device__ __forceinline__ void publish(T* p, T v)
 // arrays [row][rhs] with row stride YS (padded against bank conflicts).
 #define YS 17
 
-// Vl[i*YS + r] <- V[cols[i] + r*ldv] for i < nv, r < nrhs (0 elsewhere in the
+// Vl[i*YS + r] <- V[col + r*ldv] for i < nv, r < nrhs (0 elsewhere in the
 // 64 x 16 block), awaiting each value; all of a thread's loads are issued
-// before it waits for any.
+// before it waits for any.  Thread t handles row i = t & 63 (right-hand
+// sides t/64 + 4k), whose global column `col` the caller has loaded ahead.
 template <typename T, typename V>
-__device__ __forceinline__ void await_rows(T* Vl, const V* Vg, int ldv, const int* cols, int nv,
-                                           int nrhs, int* err)
+__device__ __forceinline__ void await_rows(T* Vl, const V* Vg, int ldv, int col, int nv, int nrhs,
+                                           int* err)
 {
   typedef typename Sentinel<V>::U U;
   U bits[4];
@@ -422,7 +423,7 @@ __device__ __forceinline__ void await_rows(T* Vl, const V* Vg, int ldv, const in
     adr[k] = nullptr;
     bits[k] = 0;
     if (r < nrhs && i < nv) {
-      adr[k] = reinterpret_cast<U*>(const_cast<V*>(Vg + cols[i] + (long long)r * ldv));
+      adr[k] = reinterpret_cast<U*>(const_cast<V*>(Vg + col + (long long)r * ldv));
       bits[k] = __hip_atomic_load(adr[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
@@ -507,14 +508,23 @@ __global__ __launch_bounds__(256) void k_fwd_chain(const FrontDesc* __restrict__
     const int i = 16 * wv + Mfma<T>::drow(lane, k);
     w0[k] = (c16 < nrhs && i < nr) ? w[r0 + i + (long long)c16 * m] : (T)0;
   }
+  // global columns loaded ahead: the rows this lane publishes, and the row
+  // (tid & 63) of each dependency block this thread awaits
+  int pcol[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int i = 16 * wv + Mfma<T>::drow(lane, k);
+    pcol[k] = (isblk && i < kb) ? rw[r0 + i] : 0;
+  }
   if (isblk) {
     const T* iv = Inv + d.offI + (long long)tile * 4096;
     for (int e = tid; e < 4096; e += 256) It[(e >> 6) + (e & 63) * 65] = iv[e];
   }
-  acc_t D;
+  acc_t D0, D1;  // even and odd k steps: two independent MFMA chains
 #pragma unroll
-  for (int k = 0; k < 4; k++) D[k] = (T)0;
+  for (int k = 0; k < 4; k++) D0[k] = D1[k] = (T)0;
   T lv[16];
+  int dcol = 0;
   auto fetch = [&](int j) {
     const int c0 = j * 64, kbj = min(64, s - c0);
 #pragma unroll
@@ -522,22 +532,26 @@ __global__ __launch_bounds__(256) void k_fwd_chain(const FrontDesc* __restrict__
       const int k = 4 * st + kq;
       lv[st] = (arow < nr && k < kbj) ? Lf[r0 + arow + (long long)(c0 + k) * m] : (T)0;
     }
+    dcol = (tid & 63) < kbj ? rw[c0 + (tid & 63)] : 0;
   };
   if (ndep > 0) fetch(0);
   for (int j = 0; j < ndep; j++) {
     T* yb = Yl[j & 1];
-    await_rows(yb, Y, ldy, rw + 64 * j, min(64, s - 64 * j), nrhs, err);
+    await_rows(yb, Y, ldy, dcol, min(64, s - 64 * j), nrhs, err);
     __syncthreads();
     T cur[16];
 #pragma unroll
     for (int st = 0; st < 16; st++) cur[st] = lv[st];
     if (j + 1 < ndep) fetch(j + 1);
 #pragma unroll
-    for (int st = 0; st < 16; st++) D = Mfma<T>::step(cur[st], yb[(4 * st + kq) * YS + c16], D);
+    for (int st = 0; st < 16; st += 2) {
+      D0 = Mfma<T>::step(cur[st], yb[(4 * st + kq) * YS + c16], D0);
+      D1 = Mfma<T>::step(cur[st + 1], yb[(4 * st + 4 + kq) * YS + c16], D1);
+    }
   }
   T acc[4];
 #pragma unroll
-  for (int k = 0; k < 4; k++) acc[k] = w0[k] - D[k];
+  for (int k = 0; k < 4; k++) acc[k] = w0[k] - (D0[k] + D1[k]);
   if (!isblk) {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -551,16 +565,21 @@ __global__ __launch_bounds__(256) void k_fwd_chain(const FrontDesc* __restrict__
 #pragma unroll
   for (int k = 0; k < 4; k++) Ac[(16 * wv + Mfma<T>::drow(lane, k)) * YS + c16] = acc[k];
   __syncthreads();
+  acc_t Ya, Yb;
+#pragma unroll
+  for (int k = 0; k < 4; k++) Ya[k] = Yb[k] = (T)0;
+#pragma unroll
+  for (int st = 0; st < 16; st += 2) {
+    Ya = Mfma<T>::step(It[arow + (4 * st + kq) * 65], Ac[(4 * st + kq) * YS + c16], Ya);
+    Yb = Mfma<T>::step(It[arow + (4 * st + 4 + kq) * 65], Ac[(4 * st + 4 + kq) * YS + c16], Yb);
+  }
   acc_t Yd;
 #pragma unroll
-  for (int k = 0; k < 4; k++) Yd[k] = (T)0;
-#pragma unroll
-  for (int st = 0; st < 16; st++)
-    Yd = Mfma<T>::step(It[arow + (4 * st + kq) * 65], Ac[(4 * st + kq) * YS + c16], Yd);
+  for (int k = 0; k < 4; k++) Yd[k] = Ya[k] + Yb[k];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int i = 16 * wv + Mfma<T>::drow(lane, k);
-    if (i < kb && c16 < nrhs) publish(Y + rw[r0 + i] + (long long)c16 * ldy, (T)Yd[k]);
+    if (i < kb && c16 < nrhs) publish(Y + pcol[k] + (long long)c16 * ldy, (T)Yd[k]);
   }
   if (nr > kb) {  // sub-diagonal rows in the last block's tile: acc -= L[rows, blk t] y_t
     T* Yv = Yl[(ndep + 1) & 1];  // free since the barrier above
@@ -638,10 +657,17 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const FrontDesc* __restrict__
     for (int e = tid; e < 4096; e += 256) It[(e >> 6) + (e & 63) * 65] = iv[e];
   }
   const int ndep = nb - 1 - blk;  // blocks nb-1 down to blk+1
-  acc_t D;
+  int pcol[4];  // the columns this lane publishes, loaded ahead
 #pragma unroll
-  for (int k = 0; k < 4; k++) D[k] = (T)0;
+  for (int k = 0; k < 4; k++) {
+    const int q = 16 * wv + Mfma<T>::drow(lane, k);
+    pcol[k] = q < kb ? rw[c0 + q] : 0;
+  }
+  acc_t D0, D1;  // even and odd k steps: two independent MFMA chains
+#pragma unroll
+  for (int k = 0; k < 4; k++) D0[k] = D1[k] = (T)0;
   T lv[16];
+  int dcol = 0;
   auto fetch = [&](int u) {
     const int rr0 = 64 * (nb - 1 - u), nrr = min(64, s - rr0);
 #pragma unroll
@@ -649,35 +675,42 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const FrontDesc* __restrict__
       const int i = 4 * st + kq;
       lv[st] = (acol < kb && i < nrr) ? Lf[rr0 + i + (long long)(c0 + acol) * m] : (T)0;
     }
+    dcol = (tid & 63) < nrr ? rw[rr0 + (tid & 63)] : 0;
   };
   if (ndep > 0) fetch(0);
   for (int u = 0; u < ndep; u++) {
     const int rr0 = 64 * (nb - 1 - u);
     T* xb = Xl[u & 1];
-    await_rows(xb, x, ldx, rw + rr0, min(64, s - rr0), nrhs, err);
+    await_rows(xb, x, ldx, dcol, min(64, s - rr0), nrhs, err);
     __syncthreads();
     T cur[16];
 #pragma unroll
     for (int st = 0; st < 16; st++) cur[st] = lv[st];
     if (u + 1 < ndep) fetch(u + 1);
 #pragma unroll
-    for (int st = 0; st < 16; st++) D = Mfma<T>::step(cur[st], xb[(4 * st + kq) * YS + c16], D);
+    for (int st = 0; st < 16; st += 2) {
+      D0 = Mfma<T>::step(cur[st], xb[(4 * st + kq) * YS + c16], D0);
+      D1 = Mfma<T>::step(cur[st + 1], xb[(4 * st + 4 + kq) * YS + c16], D1);
+    }
   }
   T* Z = Xl[ndep & 1];  // not read since the last dependency's barrier
 #pragma unroll
-  for (int k = 0; k < 4; k++) Z[(16 * wv + Mfma<T>::drow(lane, k)) * YS + c16] = y0[k] - D[k];
+  for (int k = 0; k < 4; k++)
+    Z[(16 * wv + Mfma<T>::drow(lane, k)) * YS + c16] = y0[k] - (D0[k] + D1[k]);
   __syncthreads();
   // x = inv(L_bb)^T z: A[q][p] = inv(p, q)
-  acc_t X;
+  acc_t Xa, Xb;
 #pragma unroll
-  for (int k = 0; k < 4; k++) X[k] = (T)0;
+  for (int k = 0; k < 4; k++) Xa[k] = Xb[k] = (T)0;
 #pragma unroll
-  for (int st = 0; st < 16; st++)
-    X = Mfma<T>::step(It[(4 * st + kq) + acol * 65], Z[(4 * st + kq) * YS + c16], X);
+  for (int st = 0; st < 16; st += 2) {
+    Xa = Mfma<T>::step(It[(4 * st + kq) + acol * 65], Z[(4 * st + kq) * YS + c16], Xa);
+    Xb = Mfma<T>::step(It[(4 * st + 4 + kq) + acol * 65], Z[(4 * st + 4 + kq) * YS + c16], Xb);
+  }
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int q = 16 * wv + Mfma<T>::drow(lane, k);
-    if (q < kb && c16 < nrhs) publish(x + rw[c0 + q] + (long long)c16 * ldx, (double)X[k]);
+    if (q < kb && c16 < nrhs) publish(x + pcol[k] + (long long)c16 * ldx, (double)(Xa[k] + Xb[k]));
   }
 }
 
