@@ -36,6 +36,7 @@ template <typename T> __global__ void k_fwd_rect(const FrontDesc*, const PanelTa
 template <typename T> __global__ void k_fwd_rect_reduce(const FrontDesc*, const PanelTask*, const T*, T*, int);
 template <typename T> __global__ void k_bwd_cols_reduce(const FrontDesc*, const PanelTask*, const int*, const T*, T*, int, int);
 template <typename T> __global__ void k_solve_ext(const FrontDesc*, const PanelTask*, const int*, T*, int);
+template <typename T> __global__ void k_solve_ext_seq(const FrontDesc*, const PanelTask*, const PanelTask*, const int*, T*, int);
 template <typename T> __global__ void k_bwd_gather(const FrontDesc*, const PanelTask*, const int*, const double*, int, T*, int);
 template <typename T> __global__ void k_bwd_cols(const FrontDesc*, const PanelTask*, const T*, const T*, T*, int);
 template <typename T> __global__ void k_bwd_step(const FrontDesc*, const PanelTask*, const int*, const T*, const T*, T*, int, double*, int, int);
@@ -74,7 +75,7 @@ struct Op {
 
 // solve op kinds
 enum { SV_INIT, SV_FSTEP, SV_FRECT, SV_FRED, SV_EXT, SV_GATHER, SV_BCOLS, SV_BRED, SV_BSTEP,
-       SV_CLEAR, SV_FCHAIN, SV_BCHAIN, SV_FSMALL, SV_BSMALL, SV_BUPD };
+       SV_CLEAR, SV_FCHAIN, SV_BCHAIN, SV_FSMALL, SV_BSMALL, SV_BUPD, SV_EXTSEQ };
 
 struct tamd_dev {
   hipStream_t stream = nullptr;
@@ -760,14 +761,43 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       if (fr[k].m > 64)
         for (int tt = nblk(k); tt < (fr[k].m + 63) / 64; tt++) svt.push_back({k, tt, 0, 0, 0});
     push_chain(SV_FCHAIN, t0, l, true);
+    // children's updates into the parents: parents whose children here all
+    // have at most 256 update rows get one workgroup each (k_solve_ext_seq,
+    // children in sibling-rank order); the others by sibling rank
+    std::vector<int> pmaxu(nf, 0);
+    for (int k = b0; k < b1; k++)
+      if (fr[k].parent >= 0) pmaxu[fr[k].parent] = std::max(pmaxu[fr[k].parent], fr[k].m - fr[k].s);
+    auto ext_grouped = [&](int p) { return pmaxu[p] <= 256; };
+    {
+      std::vector<std::vector<int>> kids(nf);
+      for (int k = b0; k < b1; k++)
+        if (fr[k].parent >= 0 && fr[k].m > fr[k].s && ext_grouped(fr[k].parent))
+          kids[fr[k].parent].push_back(k);
+      std::vector<PanelTask> heads, body;
+      for (int p = 0; p < nf; p++) {
+        if (kids[p].empty()) continue;
+        std::sort(kids[p].begin(), kids[p].end(), [&](int a, int b) { return sibr[a] < sibr[b]; });
+        heads.push_back({(int)body.size(), (int)kids[p].size(), 0, 0, 0});
+        for (int k : kids[p]) body.push_back({k, 0, 0, 0, fr[k].m - fr[k].s});
+      }
+      if (!heads.empty()) {
+        t0 = svt.size();
+        const int bb = (int)(t0 + heads.size());
+        for (PanelTask& h : heads) h.front += bb;  // heads index the bodies in the same array
+        svt.insert(svt.end(), heads.begin(), heads.end());
+        push_chain(SV_EXTSEQ, t0, l, false);
+        svt.insert(svt.end(), body.begin(), body.end());
+      }
+    }
     int maxrank = -1;
     for (int k = b0; k < b1; k++)
-      if (fr[k].parent >= 0 && fr[k].m > fr[k].s) maxrank = std::max(maxrank, sibr[k]);
+      if (fr[k].parent >= 0 && fr[k].m > fr[k].s && !ext_grouped(fr[k].parent))
+        maxrank = std::max(maxrank, sibr[k]);
     for (int rk = 0; rk <= maxrank; rk++) {
       t0 = svt.size();
       for (int k = b0; k < b1; k++) {
         int u = fr[k].m - fr[k].s;
-        if (fr[k].parent < 0 || u == 0 || sibr[k] != rk) continue;
+        if (fr[k].parent < 0 || u == 0 || sibr[k] != rk || ext_grouped(fr[k].parent)) continue;
         for (int r0 = 0; r0 < u; r0 += 256) svt.push_back({k, 0, 0, r0, std::min(256, u - r0)});
       }
       push_chain(SV_EXT, t0, l, false);
@@ -1399,6 +1429,10 @@ static int emit_solve(tamd_dev* D, Emitter& E, int n, int nrhs, double* dX, int 
              nrhs);
         break;
       }
+      case SV_EXTSEQ:
+        EMIT(0, true, k_solve_ext_seq<T>, grid, dim3(256), fr, tk, (const PanelTask*)D->d_sv_tasks,
+             (const int*)D->d_rel, W, nrhs);
+        break;
       case SV_BUPD:
         EMIT(0, true, k_bwd_upd<T>, grid, dim3(256), fr, tk, rows, L, (const double*)dX, ldx, Part, nrhs);
         break;

@@ -228,6 +228,32 @@ __global__ void k_solve_ext(const FrontDesc* __restrict__ fr, const PanelTask* _
   for (int q = 0; q < nrhs; q++) wp[pi + (long long)q * p.m] += wc[i + (long long)q * c.m];
 }
 
+// The same for parents whose children at this level are all small: one
+// workgroup per parent, heads[b] = {front: first task, k0: count}, the
+// children's tasks (whole update parts, nr <= blockDim) added in sibling-rank
+// order, so each parent row receives its children's values in the same order
+// as under the per-rank launches.
+template <typename T>
+__global__ void k_solve_ext_seq(const FrontDesc* __restrict__ fr, const PanelTask* __restrict__ heads,
+                                const PanelTask* __restrict__ tasks, const int* __restrict__ rel,
+                                T* __restrict__ W, int nrhs)
+{
+  const PanelTask h = heads[blockIdx.x];
+  for (int q = 0; q < h.k0; q++) {
+    if (q > 0) __syncthreads();
+    const PanelTask t = tasks[h.front + q];
+    const FrontDesc c = fr[t.front];
+    const FrontDesc p = fr[c.parent];
+    const int* r = rel + c.rel;
+    T* wp = W + p.offW * nrhs;
+    const T* wc = W + c.offW * nrhs + c.s;
+    for (int i = threadIdx.x; i < t.nr; i += blockDim.x) {
+      const int pi = r[i];
+      for (int k = 0; k < nrhs; k++) wp[pi + (long long)k * p.m] += wc[i + (long long)k * c.m];
+    }
+  }
+}
+
 // Backward: W_f[s + i] <- x(update row i) for rows r0..r0+nr-1 of the update part.
 template <typename T>
 __global__ void k_bwd_gather(const FrontDesc* __restrict__ fr, const PanelTask* __restrict__ tasks,
@@ -900,6 +926,8 @@ __global__ __launch_bounds__(256) void k_bwd_small(const FrontDesc* __restrict__
   template __global__ void k_bwd_cols_reduce<T>(const FrontDesc*, const PanelTask*, const int*,  \
                                                 const T*, T*, int, int);                        \
   template __global__ void k_solve_ext<T>(const FrontDesc*, const PanelTask*, const int*, T*, int); \
+  template __global__ void k_solve_ext_seq<T>(const FrontDesc*, const PanelTask*, const PanelTask*, \
+                                              const int*, T*, int);                             \
   template __global__ void k_bwd_gather<T>(const FrontDesc*, const PanelTask*, const int*,        \
                                            const double*, int, T*, int);                        \
   template __global__ void k_bwd_cols<T>(const FrontDesc*, const PanelTask*, const T*, const T*, \
