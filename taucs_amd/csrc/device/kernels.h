@@ -8,8 +8,10 @@
 //      blocks of disjoint lifetimes share space, each is zeroed (ZeroTask)
 //      at the start of the level where its first writer runs.
 //   W  solve workspace: m_f x nrhs per front.
-// Work is described by task arrays built once on the host per level schedule;
-// every kernel reads its task by blockIdx and has no inter-block dependence.
+// Work is described by task arrays built once on the host per level schedule.
+// A kernel reads its task by blockIdx, except the chained solve sweeps, whose
+// tasks take a ticket and wait (bounded) for values that earlier tickets of
+// the same launch publish.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
