@@ -134,6 +134,33 @@ void              taucs_dsupernodal_factor_free_numeric(void* L);
 taucs_ccs_matrix* taucs_dsupernodal_factor_to_ccs(void* L);
 taucs_double*     taucs_dsupernodal_factor_get_diag(void* L);
 
+/* src/taucs_ccs_solve_llt.c:19-130, :420-447.  Solves L L^T x = b with a
+   lower-triangular CCS factor whose columns start with their diagonal entry
+   (the export of taucs_supernodal_factor_to_ccs).  Both sweeps run on the
+   GPU; x, b are host arrays of length n.  Returns 0, or -1 with a message
+   for a factor that is not lower triangular, not double, or malformed. */
+int taucs_ccs_solve_llt(void* L, void* x, void* b);
+
+/* ---- on-disk formats (host) --------------------------------------------- */
+
+/* src/taucs_ccs_io.c:34-121 (binary: nrows, ncols, flags, colptr, rowind,
+   values); the writer is this library's own, in the same format */
+taucs_ccs_matrix* taucs_ccs_read_binary(char* filename);
+int               taucs_ccs_write_binary(taucs_ccs_matrix* A, char* filename);
+/* src/taucs_ccs_io.c:280-575 ("i j v" text, 1-based) */
+int               taucs_ccs_write_ijv(taucs_ccs_matrix* A, char* filename);
+taucs_ccs_matrix* taucs_ccs_read_ijv(char* filename, int flags);
+/* src/taucs_ccs_io.c:580-820 (MatrixMarket coordinate entries) */
+taucs_ccs_matrix* taucs_ccs_read_mtx(char* filename, int flags);
+/* src/taucs_ccs_io.c:826-960 (N, colptr, 0-based rowind, values) */
+taucs_ccs_matrix* taucs_ccs_read_ccs(char* filename, int flags);
+/* src/taucs_ccs_io.c:128-275; the Harwell-Boeing reader is not linked, so
+   this returns NULL, as the reference does without it */
+taucs_ccs_matrix* taucs_ccs_read_hb(char* filename, int flags);
+/* src/taucs_ccs_io.c:967-1047 (n raw values) */
+void* taucs_vec_read_binary(int n, int flags, char* filename);
+int   taucs_vec_write_binary(int n, int flags, void* v, char* filename);
+
 /* ---- unified solver ----------------------------------------------------- */
 
 /* src/taucs_linsolve.c:41-151 */

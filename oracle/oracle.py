@@ -257,6 +257,25 @@ def solve(sym, panels, b):
     return out.reshape(b.shape)
 
 
+def ccs_solve_llt(n, colptr, rowind, vals, b):
+    """x = (L L^T)^-1 b for a lower CSC factor whose columns start with their
+    diagonal (src/taucs_ccs_solve_llt.c:19-130): forward by columns, each
+    column scattering y_j L_ij into the rows below it; backward by columns
+    from the last, each gathering L_ji x_j before dividing by L_ii."""
+    x = np.array(b, dtype=np.float64)
+    y = np.zeros(n)
+    for j in range(n):
+        p0, p1 = colptr[j], colptr[j + 1]
+        assert rowind[p0] == j
+        y[j] = x[j] / vals[p0]
+        x[rowind[p0 + 1:p1]] -= y[j] * vals[p0 + 1:p1]
+    for i in range(n - 1, -1, -1):
+        p0, p1 = colptr[i], colptr[i + 1]
+        y[i] -= np.dot(x[rowind[p0 + 1:p1]], vals[p0 + 1:p1])
+        x[i] = y[i] / vals[p0]
+    return x
+
+
 def residual(n, colptr, rowind, vals, x, b):
     """b - A x for symmetric lower CSC (src/taucs_ccs_ops.c:299-344)."""
     import scipy.sparse as sp

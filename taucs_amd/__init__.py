@@ -21,6 +21,7 @@ lib = C.CDLL(LIB_PATH)
 TAUCS_SUCCESS, TAUCS_ERROR, TAUCS_ERROR_NOMEM, TAUCS_ERROR_BADARGS = 0, -1, -2, -3
 TAUCS_INT, TAUCS_DOUBLE, TAUCS_SINGLE = 1024, 2048, 4096
 TAUCS_LOWER, TAUCS_UPPER, TAUCS_TRIANGULAR, TAUCS_SYMMETRIC = 1, 2, 4, 8
+TAUCS_HERMITIAN, TAUCS_PATTERN = 16, 32
 FP64, FP32, BF16UPD = 0, 1, 2
 
 
@@ -76,6 +77,16 @@ _sigs = {
     "taucs_supernodal_factor_free_numeric": (None, [C.c_void_p]),
     "taucs_supernodal_factor_to_ccs": (_ccs_p, [C.c_void_p]),
     "taucs_supernodal_factor_get_diag": (C.c_void_p, [C.c_void_p]),
+    "taucs_ccs_solve_llt": (C.c_int, [_ccs_p, C.c_void_p, C.c_void_p]),
+    "taucs_ccs_read_binary": (_ccs_p, [C.c_char_p]),
+    "taucs_ccs_write_binary": (C.c_int, [_ccs_p, C.c_char_p]),
+    "taucs_ccs_write_ijv": (C.c_int, [_ccs_p, C.c_char_p]),
+    "taucs_ccs_read_ijv": (_ccs_p, [C.c_char_p, C.c_int]),
+    "taucs_ccs_read_mtx": (_ccs_p, [C.c_char_p, C.c_int]),
+    "taucs_ccs_read_ccs": (_ccs_p, [C.c_char_p, C.c_int]),
+    "taucs_ccs_read_hb": (_ccs_p, [C.c_char_p, C.c_int]),
+    "taucs_vec_read_binary": (C.c_void_p, [C.c_int, C.c_int, C.c_char_p]),
+    "taucs_vec_write_binary": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_char_p]),
     "taucs_linsolve": (C.c_int, [_ccs_p, P(C.c_void_p), C.c_int, C.c_void_p, C.c_void_p,
                                  P(C.c_char_p), P(C.c_void_p)]),
     "taucs_logfile": (None, [C.c_char_p]),
@@ -182,6 +193,23 @@ class Matrix:
         lib.taucs_ccs_times_vec(self.ptr, _dptr(x), _dptr(y))
         return y
 
+    def solve_llt(self, b):
+        """x with L L^T x = b for a lower-triangular factor L (this matrix, as
+        taucs_supernodal_factor_to_ccs exports it), both sweeps on the GPU:
+        taucs_ccs_solve_llt."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty_like(b)
+        rc = lib.taucs_ccs_solve_llt(self.ptr, _dptr(x), _dptr(b))
+        if rc != 0:
+            raise RuntimeError(f"taucs_ccs_solve_llt failed ({rc})")
+        return x
+
+    def write(self, path, fmt="binary"):
+        """Write in the binary or ijv format (taucs_ccs_write_binary/_ijv)."""
+        f = {"binary": lib.taucs_ccs_write_binary, "ijv": lib.taucs_ccs_write_ijv}[fmt]
+        if f(self.ptr, os.fsencode(path)) != 0:
+            raise RuntimeError(f"writing {path} failed")
+
     def free(self):
         if self.ptr:
             lib.taucs_ccs_free(self.ptr)
@@ -204,6 +232,34 @@ def from_arrays(n, colptr, rowind, vals, flags=TAUCS_SYMMETRIC | TAUCS_LOWER | T
     C.memmove(A.rowind, np.ascontiguousarray(rowind, dtype=np.int32).ctypes.data, 4 * nnz)
     C.memmove(A.values, np.ascontiguousarray(vals, dtype=np.float64).ctypes.data, 8 * nnz)
     return Matrix(p)
+
+
+def read_matrix(path, fmt="binary", flags=TAUCS_SYMMETRIC | TAUCS_LOWER | TAUCS_DOUBLE):
+    """Read a matrix file: fmt is "binary" (the flags are in the file),
+    "ijv", "mtx", "ccs" or "hb" (taucs_ccs_read_*)."""
+    name = os.fsencode(path)
+    if fmt == "binary":
+        p = lib.taucs_ccs_read_binary(name)
+    else:
+        p = getattr(lib, "taucs_ccs_read_" + fmt)(name, flags)
+    if not p:
+        raise RuntimeError(f"reading {path} as {fmt} failed")
+    return Matrix(p)
+
+
+def write_vector(path, v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    if lib.taucs_vec_write_binary(v.size, TAUCS_DOUBLE, _dptr(v), os.fsencode(path)) != 0:
+        raise RuntimeError(f"writing {path} failed")
+
+
+def read_vector(path, n):
+    p = lib.taucs_vec_read_binary(n, TAUCS_DOUBLE, os.fsencode(path))
+    if not p:
+        raise RuntimeError(f"reading {path} failed")
+    v = np.ctypeslib.as_array(C.cast(p, P(C.c_double)), shape=(n,)).copy()
+    lib.taucs_free_stub(p)
+    return v
 
 
 def mesh2d(n, which="neumann"):

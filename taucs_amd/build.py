@@ -14,9 +14,10 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(ROOT, "_build", "obj")
 LIB = os.path.join(HERE, "libtaucs_amd.so")
 
-HOST_SOURCES = ["util.c", "ccs.c", "mmd.c", "symbolic.c", "order.c", "factor.c", "linsolve.c",
+HOST_SOURCES = ["util.c", "ccs.c", "mmd.c", "symbolic.c", "order.c", "factor.c", "linsolve.c", "io.c",
                 "batch.c"]
-DEVICE_SOURCES = ["factor_kernels.hip", "solve_kernels.hip", "driver.hip", "refine.hip"]
+DEVICE_SOURCES = ["factor_kernels.hip", "solve_kernels.hip", "driver.hip", "refine.hip",
+                  "ccs_solve.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -1,0 +1,188 @@
+// taucs_ccs_solve_llt: solve L L^T x = b with a Cholesky factor in CCS form
+// (the export of taucs_supernodal_factor_to_ccs, or any lower-triangular CCS
+// matrix whose columns start with their diagonal entry), on the GPU.
+// Reference: src/taucs_ccs_solve_llt.c:19-130 and :420-447 (host, column by
+// column; only the double datatype is computed here).
+//
+// Both sweeps run sync-free, one wavefront per row (forward) or column
+// (backward), with the readiness protocol of kernels.h: y and x start as the
+// signaling-NaN pattern and every value is published once.
+//   forward, over the rows of L (a transposed copy, CSR), in ticket order
+//     i = 0, 1, ...: y_i = (b_i - sum_{k<i} L_ik y_k) / L_ii, each lane
+//     awaiting the y_k of its entries;
+//   backward, over the columns of L, in ticket order i = n-1, n-2, ...:
+//     x_i = (y_i - sum_{j>i} L_ji x_j) / L_ii.
+// A task waits only for tasks with smaller tickets, which are running, so
+// the sweeps always finish; each lane sums its entries in order and the wave
+// sum has a fixed tree, so the result is deterministic.
+#include "kernels.h"
+#include "taucs.h"
+
+#include <algorithm>
+#include <vector>
+
+extern "C" {
+void* taucs_amd_device_malloc(size_t bytes);
+void taucs_amd_device_free(void* p);
+int taucs_amd_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int taucs_amd_memcpy_d2h(void* dst, const void* src, size_t bytes);
+}
+
+namespace tamd {
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ void k_ccs_fill(double* __restrict__ a, long long n, int* __restrict__ words, int nw)
+{
+  const double s = __builtin_bit_cast(double, Sentinel<double>::bits);
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x)
+    a[e] = s;
+  if (blockIdx.x == 0)
+    for (int e = threadIdx.x; e < nw; e += blockDim.x) words[e] = 0;
+}
+
+// rows of L: rp/ci/cv (columns increasing, the diagonal last)
+__global__ __launch_bounds__(64) void k_ccs_fwd(int n, const int* __restrict__ rp,
+                                                const int* __restrict__ ci,
+                                                const double* __restrict__ cv,
+                                                const double* __restrict__ b, double* y,
+                                                int* ticket, int* err)
+{
+  __shared__ int sh;
+  const int lane = threadIdx.x;
+  if (lane == 0) sh = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int i = sh;
+  if (i >= n) return;
+  const int p0 = rp[i], p1 = rp[i + 1] - 1;  // p1: the diagonal
+  double acc = 0.0;
+  for (int p = p0 + lane; p < p1; p += 64) acc += cv[p] * await_value(y + ci[p], err);
+  acc = wave_sum(acc);
+  if (lane == 0) publish(y + i, (b[i] - acc) / cv[p1]);
+}
+
+// columns of L (the diagonal first)
+__global__ __launch_bounds__(64) void k_ccs_bwd(int n, const int* __restrict__ cp,
+                                                const int* __restrict__ ri,
+                                                const double* __restrict__ v,
+                                                const double* __restrict__ y, double* x,
+                                                int* ticket, int* err)
+{
+  __shared__ int sh;
+  const int lane = threadIdx.x;
+  if (lane == 0) sh = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int i = n - 1 - sh;
+  if (i < 0) return;
+  const int p0 = cp[i], p1 = cp[i + 1];
+  double acc = 0.0;
+  for (int p = p0 + 1 + lane; p < p1; p += 64) acc += v[p] * await_value(x + ri[p], err);
+  acc = wave_sum(acc);
+  if (lane == 0) publish(x + i, (y[i] - acc) / v[p0]);
+}
+
+}  // namespace tamd
+
+using namespace tamd;
+
+extern "C" int taucs_ccs_solve_llt(void* vL, void* vx, void* vb)
+{
+  taucs_ccs_matrix* L = (taucs_ccs_matrix*)vL;
+  double* x = (double*)vx;
+  const double* b = (const double*)vb;
+  if (!L || !x || !b) return -1;
+  if (!(L->flags & TAUCS_TRIANGULAR)) {
+    taucs_printf((char*)"taucs_ccs_solve_llt: factor matrix must be triangular\n");
+    return -1;
+  }
+  if (!(L->flags & TAUCS_LOWER)) {
+    taucs_printf((char*)"taucs_ccs_solve_llt: lower part must be represented\n");
+    return -1;
+  }
+  if (!(L->flags & TAUCS_DOUBLE)) {
+    taucs_printf((char*)"taucs_ccs_solve_llt: only double factors are supported\n");
+    return -1;
+  }
+  const int n = L->n;
+  if (n <= 0) return 0;
+  const int nnz = L->colptr[n];
+  // every column starts with its diagonal; the other rows are below it
+  for (int j = 0; j < n; j++) {
+    const int p0 = L->colptr[j], p1 = L->colptr[j + 1];
+    if (p1 <= p0 || L->rowind[p0] != j) {
+      taucs_printf((char*)"taucs_ccs_solve_llt: column %d does not start with its diagonal\n", j);
+      return -1;
+    }
+    for (int p = p0 + 1; p < p1; p++)
+      if (L->rowind[p] <= j || L->rowind[p] >= n) {
+        taucs_printf((char*)"taucs_ccs_solve_llt: column %d has an entry outside the lower triangle\n", j);
+        return -1;
+      }
+  }
+  // the rows of L (transpose by counting sort; columns increase along a row,
+  // so the diagonal is each row's last entry)
+  std::vector<int> rp(n + 1, 0), ci(nnz);
+  std::vector<double> cv(nnz);
+  for (int p = 0; p < nnz; p++) rp[L->rowind[p] + 1]++;
+  for (int i = 0; i < n; i++) rp[i + 1] += rp[i];
+  {
+    std::vector<int> next(rp.begin(), rp.end() - 1);
+    for (int j = 0; j < n; j++)
+      for (int p = L->colptr[j]; p < L->colptr[j + 1]; p++) {
+        const int q = next[L->rowind[p]]++;
+        ci[q] = j;
+        cv[q] = L->values.d[p];
+      }
+  }
+  const size_t ib = sizeof(int), db = sizeof(double);
+  void* buf[10] = {nullptr};
+  buf[0] = taucs_amd_device_malloc(ib * (n + 1));   // colptr
+  buf[1] = taucs_amd_device_malloc(ib * nnz);       // rowind
+  buf[2] = taucs_amd_device_malloc(db * nnz);       // values
+  buf[3] = taucs_amd_device_malloc(ib * (n + 1));   // row pointers
+  buf[4] = taucs_amd_device_malloc(ib * nnz);       // column indices
+  buf[5] = taucs_amd_device_malloc(db * nnz);       // row values
+  buf[6] = taucs_amd_device_malloc(db * n);         // b
+  buf[7] = taucs_amd_device_malloc(db * 2 * n);     // y, x
+  buf[8] = taucs_amd_device_malloc(ib * 4);         // tickets, error word
+  int rc = 0;
+  for (int k = 0; k < 9; k++)
+    if (!buf[k]) rc = -1;
+  if (rc == 0)
+    rc = taucs_amd_memcpy_h2d(buf[0], L->colptr, ib * (n + 1)) ||
+         taucs_amd_memcpy_h2d(buf[1], L->rowind, ib * nnz) ||
+         taucs_amd_memcpy_h2d(buf[2], L->values.d, db * nnz) ||
+         taucs_amd_memcpy_h2d(buf[3], rp.data(), ib * (n + 1)) ||
+         taucs_amd_memcpy_h2d(buf[4], ci.data(), ib * nnz) ||
+         taucs_amd_memcpy_h2d(buf[5], cv.data(), db * nnz) ||
+         taucs_amd_memcpy_h2d(buf[6], b, db * n);
+  int words[4] = {0, 0, 0, 0};
+  if (rc == 0) {
+    double* y = (double*)buf[7];
+    int* w = (int*)buf[8];
+    const long long fill = 2LL * n;
+    unsigned g = (unsigned)std::min<long long>((fill + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_ccs_fill, dim3(g), dim3(256), 0, 0, y, fill, w, 4);
+    hipLaunchKernelGGL(k_ccs_fwd, dim3(n), dim3(64), 0, 0, n, (const int*)buf[3],
+                       (const int*)buf[4], (const double*)buf[5], (const double*)buf[6], y, w,
+                       w + 2);
+    hipLaunchKernelGGL(k_ccs_bwd, dim3(n), dim3(64), 0, 0, n, (const int*)buf[0],
+                       (const int*)buf[1], (const double*)buf[2], (const double*)y, y + n, w + 1,
+                       w + 2);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = -1;
+    if (rc == 0) rc = taucs_amd_memcpy_d2h(words, w, ib * 4) || taucs_amd_memcpy_d2h(x, y + n, db * n);
+    if (rc == 0 && words[2]) {
+      taucs_printf((char*)"taucs_ccs_solve_llt: a sweep timed out waiting for a value\n");
+      rc = -1;
+    }
+  }
+  for (int k = 0; k < 9; k++)
+    if (buf[k]) taucs_amd_device_free(buf[k]);
+  return rc ? -1 : 0;
+}

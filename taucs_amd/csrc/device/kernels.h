@@ -87,3 +87,48 @@ template <> struct Mfma<float> {
   }
   static __device__ int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
+
+// ---- readiness by value (the chained solves) ------------------------------
+// A buffer is filled with a signaling-NaN bit pattern before a sweep (no
+// arithmetic result is a signaling NaN); every value is then published with
+// one relaxed device-scope atomic store and read with relaxed device-scope
+// atomic loads until the pattern is gone, so each value is its own flag.
+#define SPIN_LIMIT (1 << 20)
+
+template <typename T> struct Sentinel;
+template <> struct Sentinel<double> {
+  typedef unsigned long long U;
+  static constexpr U bits = 0x7FF0DEAD0000BEEFull;
+};
+template <> struct Sentinel<float> {
+  typedef unsigned int U;
+  static constexpr U bits = 0x7F80BEEFu;
+};
+
+template <typename T>
+__device__ __forceinline__ void publish(T* p, T v)
+{
+  typedef typename Sentinel<T>::U U;
+  __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// one value, awaited (bounded: after SPIN_LIMIT polls the error word is marked
+// and the pattern is returned as it is)
+template <typename T>
+__device__ __forceinline__ T await_value(const T* p, int* err)
+{
+  typedef typename Sentinel<T>::U U;
+  U* a = reinterpret_cast<U*>(const_cast<T*>(p));
+  U v = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int it = 0;
+  while (v == Sentinel<T>::bits) {
+    if (++it > SPIN_LIMIT) {
+      atomicOr(err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    v = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return __builtin_bit_cast(T, v);
+}

@@ -408,25 +408,8 @@ __global__ __launch_bounds__(256) void k_bwd_step(const FrontDesc* __restrict__ 
 // Launches whose tasks wait for nothing run with ticket == nullptr (task =
 // blockIdx).
 
-#define SPIN_LIMIT (1 << 20)
 
-template <typename T> struct Sentinel;
-template <> struct Sentinel<double> {
-  typedef unsigned long long U;
-  static constexpr U bits = 0x7FF0DEAD0000BEEFull;
-};
-template <> struct Sentinel<float> {
-  typedef unsigned int U;
-  static constexpr U bits = 0x7F80BEEFu;
-};
-
-template <typename T>
-__device__ __forceinline__ void publish(T* p, T v)
-{
-  typedef typename Sentinel<T>::U U;
-  __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
+// (Sentinel<T>, publish: the readiness protocol, in kernels.h)
 
 // Right-hand-side blocks of the chained kernels live in LDS as 64 x 16
 // arrays [row][rhs] with row stride YS (padded against bank conflicts).

@@ -114,3 +114,26 @@ def test_device_solve_in_place():
     assert np.array_equal(dB.get(), X)
     dB.free()
     F.free()
+
+
+@pytest.mark.parametrize("mesh", [1, 2, 9, 14])
+def test_ccs_solve_llt(mesh):
+    """taucs_ccs_solve_llt on the CCS export of the factor: equals the oracle's
+    column sweeps (src/taucs_ccs_solve_llt.c:19-130) and the supernodal solve
+    to rounding (rtol 1e-12), and is deterministic."""
+    from oracle import oracle as O
+    print(f"case: ccs solve, mesh3d {mesh}", flush=True)
+    A = T.mesh3d(mesh)
+    F, PA = factored(A, T.FP64, True)
+    L = F.to_ccs()
+    n, cp, ri, v = L.arrays()
+    assert np.array_equal(ri[cp[:-1]], np.arange(n))
+    b = T.random_uniform(n, 3 + mesh)
+    x = L.solve_llt(b)
+    xo = O.ccs_solve_llt(n, cp, ri, v, b)
+    assert np.abs(x - xo).max() <= 1e-12 * np.abs(xo).max()
+    assert np.abs(x - F.solve(b)).max() <= 1e-12 * np.abs(xo).max()
+    assert np.array_equal(L.solve_llt(b), x)
+    L.free()
+    F.free()
+

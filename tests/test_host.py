@@ -329,3 +329,23 @@ def test_update_blocks_with_overlapping_lifetimes_do_not_share_memory(kind, side
     assert all(a[1] == b[0] for a, b in zip(pl, pl[1:]))
     # reuse pays: far below the sum of all update blocks
     assert U_elems < sum(int(x) * int(x) for x in u[live])
+
+
+def test_ccs_solve_llt_rejects_non_factors():
+    """taucs_ccs_solve_llt checks flags and structure on the host before it
+    touches the device (src/taucs_ccs_solve_llt.c:30-37 checks the flags; the
+    reference asserts the diagonal-first order, here it is an error): a
+    symmetric matrix, an upper one, a lower one whose columns end with their
+    diagonal (the generators' order) and one with an entry above the diagonal
+    all return -1."""
+    A = T.mesh2d(4)
+    n, cp, ri, v = A.arrays()
+    low = T.TAUCS_TRIANGULAR | T.TAUCS_LOWER | T.TAUCS_DOUBLE
+    cases = [A, T.from_arrays(n, cp, ri, v, T.TAUCS_TRIANGULAR | T.TAUCS_UPPER | T.TAUCS_DOUBLE),
+             T.from_arrays(n, cp, ri, v, low)]
+    # diagonal first, but column 1 also holds row 0
+    cp3 = np.array([0, 1, 3]); ri3 = np.array([0, 1, 0]); v3 = np.array([2.0, 2.0, 1.0])
+    cases.append(T.from_arrays(2, cp3, ri3, v3, low))
+    for M in cases:
+        x = np.zeros(M.n)
+        assert T.lib.taucs_ccs_solve_llt(M.ptr, T._dptr(x), T._dptr(np.ones(M.n))) == -1

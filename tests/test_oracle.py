@@ -18,6 +18,9 @@ mistake fails:
                   published thresholds
   numeric      -- factor equals numpy.linalg.cholesky of P A P^T; solve equals
                   numpy.linalg.solve
+  CCS solve    -- equals scipy's dense triangular solves L^-T L^-1 b on a
+                  random sparse factor (a transposed sweep or a dropped
+                  diagonal fails it)
 """
 import numpy as np
 import pytest
@@ -244,3 +247,22 @@ def test_numeric_factor_rejects_indefinite():
     sym = O.symbolic(n, cp, ri)
     with pytest.raises(np.linalg.LinAlgError):
         O.factor(sym, cp, ri, v)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_ccs_solve_llt_matches_dense_triangular_solves(seed):
+    import scipy.linalg as sl
+    rng = np.random.default_rng(seed)
+    n = 60
+    L = np.tril(rng.uniform(-1, 1, (n, n)) * (rng.uniform(0, 1, (n, n)) < 0.15), -1)
+    L += np.diag(rng.uniform(1, 3, n))
+    cp, ri, v = [0], [], []
+    for j in range(n):                 # diagonal first, then the rows below
+        rows = [j] + [i for i in range(j + 1, n) if L[i, j] != 0]
+        ri += rows
+        v += [L[i, j] for i in rows]
+        cp.append(len(ri))
+    b = rng.uniform(-1, 1, n)
+    x = O.ccs_solve_llt(n, np.array(cp), np.array(ri), np.array(v), b)
+    xd = sl.solve_triangular(L, sl.solve_triangular(L, b, lower=True), lower=True, trans="T")
+    assert np.abs(x - xd).max() <= 1e-12 * np.abs(xd).max()
