@@ -25,6 +25,11 @@ ARCH = "gfx950"
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wno-unused-result",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include")]
+# Per-file additions.  The factor kernels keep MFMA accumulators in
+# ordinary VGPRs: a 64x64 FP64 tile wave then needs 104 registers instead of
+# 128, so two of them leave room on a SIMD for the 288-register panel POTRF
+# wave, which must run beside the trailing update.
+FILE_FLAGS = {"factor_kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu99", "-Wall", "-Wno-unused-function",
            "-I" + os.path.join(ROOT, "include")]
 
@@ -48,7 +53,7 @@ def build(verbose=False):
     for src in DEVICE_SOURCES:
         s = os.path.join(CSRC, "device", src)
         o = os.path.join(OBJ, src + ".o")
-        jobs.append([HIPCC] + HIP_FLAGS + ["-c", s, "-o", o])
+        jobs.append([HIPCC] + HIP_FLAGS + FILE_FLAGS.get(src, []) + ["-c", s, "-o", o])
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         outs = list(ex.map(_run, jobs))
     if verbose:

@@ -1024,6 +1024,7 @@ struct Emitter {
   hipGraphNode_t last_main = nullptr, last_side = nullptr;
   bool side_pending = false;
   int count = 0;
+  unsigned shmem = 0;  // dynamic LDS of the next launch (then reset)
 
   template <typename... P, size_t... I>
   static void ptrs_of(std::tuple<P...>& t, void** p, std::index_sequence<I...>)
@@ -1038,6 +1039,8 @@ struct Emitter {
     void* p[sizeof...(P) > 0 ? sizeof...(P) : 1];
     ptrs_of(vals, p, std::index_sequence_for<P...>{});
     count++;
+    const unsigned lds = shmem;
+    shmem = 0;
     if (g) {
       hipGraphNode_t deps[2];
       int nd = 0;
@@ -1053,7 +1056,7 @@ struct Emitter {
       kp.func = (void*)k;
       kp.gridDim = grid;
       kp.blockDim = block;
-      kp.sharedMemBytes = 0;
+      kp.sharedMemBytes = lds;
       kp.kernelParams = p;
       kp.extra = nullptr;
       hipGraphNode_t node;
@@ -1075,7 +1078,7 @@ struct Emitter {
       HIPCHECK(hipStreamWaitEvent(main_st, ev_join, 0));
       side_pending = false;
     }
-    HIPCHECK(hipLaunchKernel((const void*)k, grid, block, p, 0, st));
+    HIPCHECK(hipLaunchKernel((const void*)k, grid, block, p, lds, st));
     if (lane == 1 && side_st) {
       HIPCHECK(hipEventRecord(ev_join, side_st));
       side_pending = true;
@@ -1098,6 +1101,19 @@ struct Emitter {
   do {                                       \
     if (E.kernel(__VA_ARGS__) != 0) return -1; \
   } while (0)
+
+// LDS per CU is 160 KiB.  Side-lane tiles: FP64 two per CU (waves of 104
+// registers, leaving 304 on each SIMD for the 288-register POTRF wave and
+// 48 KiB of LDS for its 37 KiB); FP32 four per CU (64-register waves, POTRF
+// 152 registers and 18.5 KiB).
+template <typename T> constexpr int SIDE_WGS = sizeof(T) == 8 ? 2 : 4;
+template <typename T>
+static unsigned side_lds_pad()
+{
+  const unsigned tile_lds = 2u * 64 * (16 + 2) * sizeof(T);  // As + Bs of the 64x64 tile
+  const unsigned per_wg = (160u * 1024 - (sizeof(T) == 8 ? 48u : 24u) * 1024) / SIDE_WGS<T>;
+  return per_wg > tile_lds ? per_wg - tile_lds : 0;
+}
 
 template <typename T>
 static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
@@ -1156,9 +1172,15 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
         else if (op.tile == 128)
           EMIT(lane, join, k_gemm_tile128<T>, dim3(op.count), dim3(256),
                (const GemmTask*)(D->d_gemm + op.start), (const T*)L, L, U);
-        else
+        else {
+          // The lookahead's trailing update reserves LDS it does not use, so
+          // that at most SIDE_WGS<T> of its workgroups share a CU: the panel
+          // chain's POTRF, TRSM and inner updates then find room beside it
+          // instead of waiting for the whole update to drain.
+          if (lane == 1) E.shmem = side_lds_pad<T>();
           EMIT(lane, join, k_gemm_tile<T>, dim3(op.count), dim3(256),
                (const GemmTask*)(D->d_gemm + op.start), (const T*)L, L, U, (const T*)Iv);
+        }
         break;
       case TAMD_OP_EXTEND:
         if (op.grouped)

@@ -369,6 +369,9 @@ __global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__
   __shared__ __align__(16) T Ls[64 * PW_LD];
   __shared__ __align__(16) T rdiag[64];
   __shared__ __align__(16) T Ts[16 * 17];
+  // the panel chain is the critical path: its waves win issue arbitration
+  // against the trailing-update waves that share their SIMD
+  __builtin_amdgcn_s_setprio(3);
   const PanelTask t = tasks[blockIdx.x];
   const FrontDesc d = fr[t.front];
   const long long m = d.m;
@@ -583,6 +586,7 @@ __global__ __launch_bounds__(256) void k_trsm_tile(const GemmTask* __restrict__ 
                                                    const T* Lsrc, T* L, T* U,
                                                    const T* __restrict__ Inv)
 {
+  __builtin_amdgcn_s_setprio(2);
   gemm_tile<T, 1>(tasks, Lsrc, L, U, Inv);
 }
 
