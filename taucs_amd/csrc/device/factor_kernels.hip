@@ -234,12 +234,30 @@ __device__ __forceinline__ float rlane<float>(float v, int l)
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// d = sqrt(p) and 1/d, both correctly rounded (as the reference's dpotrf)
+// d = sqrt(p) and rd = 1/d for a pivot p > 0.  FP32: correctly rounded.
+// FP64: Newton's iteration for y = 1/sqrt(p), y <- y + y (1 - p y^2) / 2,
+// twice from the hardware estimate (each step squares the relative error),
+// then d = p y corrected once by the same iteration for sqrt,
+// d <- d + y (p - d^2) / 2.  Both end within about one ulp, and the chain is
+// half as long as a correctly rounded sqrt followed by a division: the pivot
+// step is the latency of the whole block.
 template <typename T>
 __device__ __forceinline__ void sqrt_pair(T p, T& d, T& rd)
 {
   d = sqrt(p);
   rd = (T)1 / d;
+}
+template <>
+__device__ __forceinline__ void sqrt_pair<double>(double p, double& d, double& rd)
+{
+  double y = __builtin_amdgcn_rsq(p);
+  double h = __builtin_fma(-p * y, y, 1.0);
+  y = __builtin_fma(0.5 * y, h, y);
+  h = __builtin_fma(-p * y, y, 1.0);
+  y = __builtin_fma(0.5 * y, h, y);
+  const double s = p * y;
+  d = __builtin_fma(0.5 * y, __builtin_fma(-s, s, p), s);
+  rd = y;
 }
 
 __device__ __forceinline__ void wave_lds_fence()

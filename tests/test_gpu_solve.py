@@ -137,3 +137,31 @@ def test_ccs_solve_llt(mesh):
     L.free()
     F.free()
 
+
+
+@pytest.mark.parametrize("width", [128, 512])
+def test_lookahead_factor_is_bitwise_identical(width):
+    """The lookahead runs each outer panel's trailing update beside the next
+    panel's POTRF/TRSM/update chain (the update's workgroups reserve LDS so
+    the chain fits beside them).  The two touch disjoint columns, so the
+    factor must be bitwise the one of the serial schedule, and a repeat
+    bitwise the same: a race would show up here first (mesh3d 44: fronts of
+    thousands of rows, trailing updates of thousands of tiles)."""
+    print(f"case: lookahead bitwise, mesh3d 44, width {width}", flush=True)
+    A = T.mesh3d(44)
+    perm, inv = T.order(A, "genmmd")
+    PA = T.permute(A, perm, inv)
+    vals = []
+    try:
+        T.lib.taucs_amd_set_panel_width(width)
+        for la in (1, 1, 0):
+            T.lib.taucs_amd_set_lookahead(la)
+            F = T.factor_symbolic(PA)
+            assert F.numeric(PA, T.FP64) == 0
+            vals.append(F.to_ccs().arrays()[3])
+            F.free()
+    finally:
+        T.lib.taucs_amd_set_lookahead(1)
+        T.lib.taucs_amd_set_panel_width(256)
+    assert np.array_equal(vals[0], vals[1])
+    assert np.array_equal(vals[0], vals[2])
