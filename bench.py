@@ -250,6 +250,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--panel-width", type=int, default=0,
                     help="use this outer panel width instead of running the autotuner")
+    ap.add_argument("--gemm-tile", type=int, default=0,
+                    help="with --panel-width: the tile edge (64 or 128) of the trailing updates")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed steps, copy this many MiB device to device once: "
                          "the reference point for hardware byte counters under a profiler")
@@ -273,7 +275,8 @@ def main():
     # the schedule autotuner picks the outer panel width (timed numeric runs)
     if args.panel_width:
         T.lib.taucs_amd_set_panel_width(args.panel_width)
-        tuned_w, tune_ms = args.panel_width, {}
+        T.lib.taucs_amd_set_gemm_tile(args.gemm_tile or 64)
+        tuned_w, tune_ms = (args.panel_width, args.gemm_tile or 64), {}
     else:
         tuned_w, tune_ms = F.autotune(PA, T.FP64, reps=2)
     log(f"autotune: panel width {tuned_w}, candidates (ms) {tune_ms}")
@@ -354,10 +357,13 @@ def main():
     # memory traffic per launch of the tile GEMM, from the committed counter
     # passes (FETCH_SIZE / WRITE_SIZE, calibrated by a 1 GiB copy) of this workload
     traffic = None
+    sched = list(tuned_w) if isinstance(tuned_w, tuple) else [tuned_w, 64]
     tp = os.path.join(ROOT, "profiles", "c2_gemm_traffic.json")
     if side == 60 and os.path.exists(tp):
         with open(tp) as fh:
-            traffic = round(json.load(fh)["bytes_per_launch"] / 1e6, 2)
+            tj = json.load(fh)
+        if tj.get("schedule") == sched:   # counters of this schedule's launches only
+            traffic = round(tj["bytes_per_launch"] / 1e6, 2)
     gemm_rate = tm.gemm_flops / (tm.gemm_ms * 1e-3) if tm.gemm_ms > 0 else 0.0
     log(f"profile: total {tm.total_ms:.2f} ms, front {tm.front_ms:.2f} ms, gemm {tm.gemm_ms:.2f} ms "
         f"({tm.gemm_launches} launches, {gemm_rate/1e12:.2f} TFLOP/s), assembly {tm.assembly_ms:.2f} ms, "
@@ -433,7 +439,10 @@ def main():
         "backward_error": bwd,
         "rel_residual_after_1_refinement": info_ref.rel_residual,
         "mixed": mixed,
-        "roofline": {"bound": "mfma", "kernel": "k_gemm_tile<double> (v_mfma_f64_16x16x4)",
+        "roofline": {"bound": "mfma",
+                     "kernel": ("tile GEMM/SYRK launches: k_gemm_tile128<double> (trailing updates, SYRK) "
+                                "and k_gemm_tile<double> (panel-internal updates), v_mfma_f64_16x16x4"
+                                if sched[1] == 128 else "k_gemm_tile<double> (v_mfma_f64_16x16x4)"),
                      "achieved": round(gemm_rate / 1e12, 3), "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": round(gemm_rate / PEAK_FP64, 4),
                      "avg_launch_ms": round(gemm_avg_ms, 4), "launches": tm.gemm_launches,
