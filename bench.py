@@ -118,12 +118,27 @@ def bench_varcoef(T, side=80, nrhs=4, steps=2, seed=2024):
                 "iterations": max(its) if its else 0, "converged": ok,
                 "backward_error": bwd, "X": X}
 
+    # each precision runs at the schedule the autotuner picks for it
+    sched = {}
+    for name, prec in (("fp64", T.FP64), ("fp32", T.FP32), ("bf16upd", T.BF16UPD)):
+        sched[prec] = F.autotune(PA, prec, reps=1)
+        log(f"varcoef autotune {name}: {sched[prec]}")
+    out["schedules"] = {name: {"panel_width": sched[p][0][0], "gemm_tile": sched[p][0][1],
+                               "autotune_ms": {f"w{w}_t{t}": round(v, 2) for (w, t), v in sched[p][1].items()}}
+                        for name, p in (("fp64", T.FP64), ("fp32", T.FP32), ("bf16upd", T.BF16UPD))}
+
+    def use(prec):
+        (w, t), _ = sched[prec]
+        F.autotune(PA, prec, candidates=((w, t),), reps=1)
+
+    use(T.FP64)
     r64 = run(T.FP64, None, 0, 0)
     out["fp64"] = r64
     x64 = r64["X"]
     base = r64["factor_ms"] + r64["solve_ms"]
     for name, prec, method in (("fp32_ir", T.FP32, 0), ("fp32_pcg", T.FP32, 1),
                                ("bf16upd_ir", T.BF16UPD, 0), ("bf16upd_pcg", T.BF16UPD, 1)):
+        use(prec)
         r = run(prec, method, 1e-12, 200)
         if r is None:
             out[name] = {"failed": "factorization broke down (pivot not positive)"}

@@ -1107,6 +1107,7 @@ struct Emitter {
 // 48 KiB of LDS for its 37 KiB); FP32 four per CU (64-register waves, POTRF
 // 152 registers and 18.5 KiB).
 template <typename T> constexpr int SIDE_WGS = sizeof(T) == 8 ? 2 : 4;
+#define SIDE_PAD_MAX_TILES_FP64 4096
 template <typename T>
 static unsigned side_lds_pad()
 {
@@ -1176,8 +1177,13 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
           // The lookahead's trailing update reserves LDS it does not use, so
           // that at most SIDE_WGS<T> of its workgroups share a CU: the panel
           // chain's POTRF, TRSM and inner updates then find room beside it
-          // instead of waiting for the whole update to drain.
-          if (lane == 1) E.shmem = side_lds_pad<T>();
+          // instead of waiting for the whole update to drain.  In FP64 only
+          // while the update is short enough for the chain to be the longer
+          // branch: a long update at half occupancy costs more than the
+          // overlap saves (measured: mesh3d 60^3 root, 2.3K tiles, gains 5%;
+          // 80^3 root, 20K tiles, loses 4%).
+          if (lane == 1 && (sizeof(T) == 4 || op.count <= SIDE_PAD_MAX_TILES_FP64))
+            E.shmem = side_lds_pad<T>();
           EMIT(lane, join, k_gemm_tile<T>, dim3(op.count), dim3(256),
                (const GemmTask*)(D->d_gemm + op.start), (const T*)L, L, U, (const T*)Iv);
         }
