@@ -274,8 +274,8 @@ def main():
                     "factorization (kind, level, tasks, flops, bytes, ms) to this JSON file")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU layout is a design in DESIGN.md, not part of this build)")
+        print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
+              "(sharding the factorization over GPUs is designed, not implemented)")
         sys.exit(2)
 
     import taucs_amd as T
