@@ -387,6 +387,7 @@ def main():
     mixed = None
     if not args.no_mixed:
         Fm = T.factor_symbolic(PA)
+        m_sched, m_tune = Fm.autotune(PA, T.FP32, reps=1)   # FP32 picks its own schedule
         tmf, tms, its, bw = [], [], [], []
         for i in range(args.warmup + args.steps):
             t0 = time.perf_counter()
@@ -401,6 +402,7 @@ def main():
                 its.append(info.iterations)
                 bw.append(info.backward_error)
         mixed = {"precision": "fp32 fronts + fp64 preconditioned CG",
+                 "schedule": {"panel_width": m_sched[0], "gemm_tile": m_sched[1]},
                  "factor_gflops": round(fi.fund_flops / np.mean(tmf) / 1e9, 2),
                  "factor_ms": round(1e3 * float(np.mean(tmf)), 3),
                  "refined_solve_ms": round(1e3 * float(np.mean(tms)), 3),
