@@ -398,8 +398,10 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
   std::vector<GemmTask> gemm;
   std::vector<EATask> ea;
   std::vector<ZeroTask> zero;
-  // update blocks by the level that zeroes them (global U offsets)
-  std::vector<std::vector<ZeroTask>> zero_at(nlev);
+  // update blocks by the level that zeroes them (global U offsets): a leaf
+  // front's own block (its first writer is its own front op or SYRK), or a
+  // parent's block (first written by that level's extend-add)
+  std::vector<std::vector<ZeroTask>> zero_at(nlev), zero_par_at(nlev);
   {
     long long ub = 0;
     for (size_t q = 0; q < sys.size(); q++) {
@@ -410,8 +412,9 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
         long long u = S->sn_up_size[sn] - S->sn_size[sn];
         if (u <= 0) continue;
         long long off = ub + P->off_U[sn], len = u * u;
+        auto& za = (P->u_start[sn] == P->sn_level[sn] ? zero_at : zero_par_at)[P->u_start[sn]];
         for (long long c = 0; c < len; c += 65536)
-          zero_at[P->u_start[sn]].push_back({off + c, std::min(65536LL, len - c)});
+          za.push_back({off + c, std::min(65536LL, len - c)});
       }
       ub += P->U_elems;
     }
@@ -427,6 +430,14 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       D->ops.push_back({TAMD_OP_ZERO, (int)zero.size(), (int)zero_at[l].size(), l});
       zero.insert(zero.end(), zero_at[l].begin(), zero_at[l].end());
     }
+    // parents' blocks are first written by this level's extend-add (which
+    // joins the side chain): their zeroing overlaps the level's fronts
+    if (!zero_par_at[l].empty()) {
+      Op o = {TAMD_OP_ZERO, (int)zero.size(), (int)zero_par_at[l].size(), l};
+      o.lane = 1;
+      D->ops.push_back(o);
+      zero.insert(zero.end(), zero_par_at[l].begin(), zero_par_at[l].end());
+    }
     int st = (int)small.size();
     int maxouter = 0;
     const int W = D->panel_w;
@@ -434,7 +445,11 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       if (fr[k].m <= SMALL_M) small.push_back(k);
       else maxouter = std::max(maxouter, (fr[k].s + W - 1) / W);
     }
-    if ((int)small.size() > st) D->ops.push_back({TAMD_OP_SMALL, st, (int)small.size() - st, l});
+    if ((int)small.size() > st) {
+      Op o = {TAMD_OP_SMALL, st, (int)small.size() - st, l};
+      o.nojoin = true;  // own blocks only: not the parents' blocks zeroed beside it
+      D->ops.push_back(o);
+    }
     // Two-level blocking: outer panels of W columns, each factored in 64-column
     // steps (POTRF of the diagonal block, TRSM of the rows below by its
     // inverse, update of the panel's remaining columns with K = 64); then one
