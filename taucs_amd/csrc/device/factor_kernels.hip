@@ -778,7 +778,26 @@ __device__ __forceinline__ void extend_add_cols(const FrontDesc* __restrict__ fr
     T* base = pj < sp ? L : U;
     const long long col0 = pj < sp ? p.offL + (long long)pj * mp
                                    : p.offU + (long long)(pj - sp) * up - sp;
-    for (int i = j + lane; i < uc; i += 64) base[col0 + r[i]] += src[i];
+    // four rows per lane per pass, all loads before the stores: the rows of a
+    // column map to distinct parent rows, and child and parent blocks never
+    // overlap, so the batch is free of conflicts (the compiler cannot know
+    // that: both may point into the update arena)
+    for (int i0 = j + lane; i0 < uc; i0 += 256) {
+      T sv[4], bv[4];
+      int ri[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int i = i0 + 64 * q;
+        sv[q] = i < uc ? src[i] : (T)0;
+        ri[q] = i < uc ? r[i] : 0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (i0 + 64 * q < uc) bv[q] = base[col0 + ri[q]];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (i0 + 64 * q < uc) base[col0 + ri[q]] = bv[q] + sv[q];
+    }
   }
 }
 
