@@ -210,6 +210,26 @@ __global__ void k_fwd_rect_reduce(const FrontDesc* __restrict__ fr,
   }
 }
 
+// dst[k * ldd] += src[k * lds] for the right-hand sides k < nrhs, eight at a
+// time with all loads before the stores (child and parent rows are distinct
+// memory, which the compiler cannot know: both lie in the workspace).
+template <typename T>
+__device__ __forceinline__ void ext_row(T* dst, int ldd, const T* src, int lds, int nrhs)
+{
+  for (int k0 = 0; k0 < nrhs; k0 += 8) {
+    T a[8], b[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (k0 + k < nrhs) {
+        a[k] = src[(long long)(k0 + k) * lds];
+        b[k] = dst[(long long)(k0 + k) * ldd];
+      }
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (k0 + k < nrhs) dst[(long long)(k0 + k) * ldd] = b[k] + a[k];
+  }
+}
+
 // Child update rows added into the parent's workspace (rows r0..r0+nr-1 of
 // the child's update part).
 template <typename T>
@@ -224,8 +244,7 @@ __global__ void k_solve_ext(const FrontDesc* __restrict__ fr, const PanelTask* _
   const T* wc = W + c.offW * nrhs + c.s;
   const int i = t.r0 + threadIdx.x;
   if (threadIdx.x >= t.nr) return;
-  const int pi = r[i];
-  for (int q = 0; q < nrhs; q++) wp[pi + (long long)q * p.m] += wc[i + (long long)q * c.m];
+  ext_row(wp + r[i], p.m, wc + i, c.m, nrhs);
 }
 
 // The same for parents whose children at this level are all small: one
@@ -247,10 +266,7 @@ __global__ void k_solve_ext_seq(const FrontDesc* __restrict__ fr, const PanelTas
     const int* r = rel + c.rel;
     T* wp = W + p.offW * nrhs;
     const T* wc = W + c.offW * nrhs + c.s;
-    for (int i = threadIdx.x; i < t.nr; i += blockDim.x) {
-      const int pi = r[i];
-      for (int k = 0; k < nrhs; k++) wp[pi + (long long)k * p.m] += wc[i + (long long)k * c.m];
-    }
+    for (int i = threadIdx.x; i < t.nr; i += blockDim.x) ext_row(wp + r[i], p.m, wc + i, c.m, nrhs);
   }
 }
 
