@@ -165,3 +165,34 @@ def test_lookahead_factor_is_bitwise_identical(width):
         T.lib.taucs_amd_set_panel_width(256)
     assert np.array_equal(vals[0], vals[1])
     assert np.array_equal(vals[0], vals[2])
+
+
+def test_matrix_file_through_the_drop_in(tmp_path):
+    """The flow of the reference's direct driver (progs/direct.c:540-550) on a
+    matrix read from a MatrixMarket file: order, factor on the GPU, solve with
+    the supernodal factor and with its CCS export (taucs_ccs_solve_llt); both
+    solutions against a dense solve of the matrix as read."""
+    print("case: mtx file -> genmmd -> factor -> supernodal and CCS solves", flush=True)
+    A = T.mesh3d(10)
+    n, cp, ri, v = A.arrays()
+    lines = [f"{n} {n} {len(v)}"]
+    for j in range(n):
+        for p in range(cp[j], cp[j + 1]):
+            lines.append(f"{ri[p] + 1} {j + 1} {float(v[p]):.17g}")
+    f = tmp_path / "mesh3d10.mtx"
+    f.write_text("%%MatrixMarket matrix coordinate real symmetric\n" + "\n".join(lines) + "\n")
+    M = T.read_matrix(f, "mtx", T.TAUCS_SYMMETRIC | T.TAUCS_LOWER | T.TAUCS_DOUBLE)
+    assert np.array_equal(M.to_scipy_full().toarray(), A.to_scipy_full().toarray())
+    perm, inv = T.order(M, "genmmd")
+    PM = T.permute(M, perm, inv)
+    F = T.factor_symbolic(PM)
+    assert F.numeric(PM, T.FP64) == 0
+    b = T.random_uniform(n, 77)
+    x1 = F.solve(b)
+    L = F.to_ccs()
+    x2 = L.solve_llt(b)
+    xd = np.linalg.solve(PM.to_scipy_full().toarray(), b)
+    for x in (x1, x2):
+        assert np.abs(x - xd).max() <= 1e-10 * np.abs(xd).max()
+    L.free()
+    F.free()
