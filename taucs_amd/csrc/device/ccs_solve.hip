@@ -14,7 +14,9 @@
 //     x_i = (y_i - sum_{j>i} L_ji x_j) / L_ii.
 // A task waits only for tasks with smaller tickets, which are running, so
 // the sweeps always finish; each lane sums its entries in order and the wave
-// sum has a fixed tree, so the result is deterministic.
+// sum has a fixed tree, so the result is deterministic.  A row loads all the
+// values it needs eight per lane at a time and waits only for those not yet
+// published (on a dense separator almost all of them were, long before).
 #include "kernels.h"
 #include "taucs.h"
 
@@ -35,6 +37,62 @@ __device__ __forceinline__ double wave_sum(double v)
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// The wait of one value, bounded by a larger limit than the supernodal
+// sweeps use: on a large factor a row can wait for a chain of thousands of
+// long rows (the root separator's), while the other waiting rows poll.
+// Tickets guarantee progress; the bound only turns a fault into an error.
+#define CCS_SPIN_LIMIT (1 << 26)
+__device__ __forceinline__ double await_long(const double* p, int* err)
+{
+  typedef Sentinel<double>::U U;
+  U* a = reinterpret_cast<U*>(const_cast<double*>(p));
+  U v = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int it = 0; v == Sentinel<double>::bits; it++) {
+    if (it > CCS_SPIN_LIMIT) {
+      atomicOr(err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(2);
+    v = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return __builtin_bit_cast(double, v);
+}
+
+// sum over p in [p0, p1) of a[p] * v[idx[p]], lane-strided: eight entries per
+// lane per pass are loaded before any is awaited, so a row waits one memory
+// round trip per pass plus the values still being computed, not one per entry
+__device__ __forceinline__ double dot_awaited(int p0, int p1, int lane, const int* __restrict__ idx,
+                                              const double* __restrict__ a, const double* v,
+                                              int* err)
+{
+  typedef Sentinel<double>::U U;
+  double acc = 0.0;
+  for (int pb = p0 + lane; pb < p1; pb += 64 * 8) {
+    U w[8];
+    double c[8];
+    int k[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int p = pb + 64 * q;
+      k[q] = p < p1 ? idx[p] : 0;
+      c[q] = p < p1 ? a[p] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      w[q] = pb + 64 * q < p1 ? __hip_atomic_load(reinterpret_cast<U*>(const_cast<double*>(v + k[q])),
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                              : (U)0;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (pb + 64 * q < p1) {
+        double x = __builtin_bit_cast(double, w[q]);
+        if (w[q] == Sentinel<double>::bits) x = await_long(v + k[q], err);
+        acc += c[q] * x;
+      }
+  }
+  return acc;
 }
 
 __global__ void k_ccs_fill(double* __restrict__ a, long long n, int* __restrict__ words, int nw)
@@ -61,9 +119,7 @@ __global__ __launch_bounds__(64) void k_ccs_fwd(int n, const int* __restrict__ r
   const int i = sh;
   if (i >= n) return;
   const int p0 = rp[i], p1 = rp[i + 1] - 1;  // p1: the diagonal
-  double acc = 0.0;
-  for (int p = p0 + lane; p < p1; p += 64) acc += cv[p] * await_value(y + ci[p], err);
-  acc = wave_sum(acc);
+  const double acc = wave_sum(dot_awaited(p0, p1, lane, ci, cv, y, err));
   if (lane == 0) publish(y + i, (b[i] - acc) / cv[p1]);
 }
 
@@ -81,9 +137,7 @@ __global__ __launch_bounds__(64) void k_ccs_bwd(int n, const int* __restrict__ c
   const int i = n - 1 - sh;
   if (i < 0) return;
   const int p0 = cp[i], p1 = cp[i + 1];
-  double acc = 0.0;
-  for (int p = p0 + 1 + lane; p < p1; p += 64) acc += v[p] * await_value(x + ri[p], err);
-  acc = wave_sum(acc);
+  const double acc = wave_sum(dot_awaited(p0 + 1, p1, lane, ri, v, x, err));
   if (lane == 0) publish(x + i, (y[i] - acc) / v[p0]);
 }
 

@@ -116,7 +116,7 @@ def test_device_solve_in_place():
     F.free()
 
 
-@pytest.mark.parametrize("mesh", [1, 2, 9, 14])
+@pytest.mark.parametrize("mesh", [1, 2, 9, 14, 30])
 def test_ccs_solve_llt(mesh):
     """taucs_ccs_solve_llt on the CCS export of the factor: equals the oracle's
     column sweeps (src/taucs_ccs_solve_llt.c:19-130) and the supernodal solve
