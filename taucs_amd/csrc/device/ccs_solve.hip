@@ -65,31 +65,38 @@ __device__ __forceinline__ double await_long(const double* p, int* err)
 
 // sum over p in [p0, p1) of a[p] * v[idx[p]], lane-strided: eight entries per
 // lane per pass are loaded before any is awaited, so a row waits one memory
-// round trip per pass plus the values still being computed, not one per entry
+// round trip per pass plus the values still being computed, not one per entry.
+// The entries are taken in the order that leaves the one computed last for
+// the end: ascending along a row of L (its last off-diagonal entry is the
+// previous row's), descending down a column (its first is the next column's),
+// so a dense chain waits only in its final pass.
+template <bool DESC>
 __device__ __forceinline__ double dot_awaited(int p0, int p1, int lane, const int* __restrict__ idx,
                                               const double* __restrict__ a, const double* v,
                                               int* err)
 {
   typedef Sentinel<double>::U U;
   double acc = 0.0;
-  for (int pb = p0 + lane; pb < p1; pb += 64 * 8) {
+  const int len = p1 - p0;
+  for (int eb = lane; eb < len; eb += 64 * 8) {
     U w[8];
     double c[8];
     int k[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      const int p = pb + 64 * q;
-      k[q] = p < p1 ? idx[p] : 0;
-      c[q] = p < p1 ? a[p] : 0.0;
+      const int e = eb + 64 * q, p = DESC ? p1 - 1 - e : p0 + e;
+      k[q] = e < len ? idx[p] : 0;
+      c[q] = e < len ? a[p] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < 8; q++)
-      w[q] = pb + 64 * q < p1 ? __hip_atomic_load(reinterpret_cast<U*>(const_cast<double*>(v + k[q])),
-                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                              : (U)0;
+      w[q] = eb + 64 * q < len
+                 ? __hip_atomic_load(reinterpret_cast<U*>(const_cast<double*>(v + k[q])),
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                 : (U)0;
 #pragma unroll
     for (int q = 0; q < 8; q++)
-      if (pb + 64 * q < p1) {
+      if (eb + 64 * q < len) {
         double x = __builtin_bit_cast(double, w[q]);
         if (w[q] == Sentinel<double>::bits) x = await_long(v + k[q], err);
         acc += c[q] * x;
@@ -122,7 +129,7 @@ __global__ __launch_bounds__(64) void k_ccs_fwd(int n, const int* __restrict__ r
   const int i = sh;
   if (i >= n) return;
   const int p0 = rp[i], p1 = rp[i + 1] - 1;  // p1: the diagonal
-  const double acc = wave_sum(dot_awaited(p0, p1, lane, ci, cv, y, err));
+  const double acc = wave_sum(dot_awaited<false>(p0, p1, lane, ci, cv, y, err));
   if (lane == 0) publish(y + i, (b[i] - acc) / cv[p1]);
 }
 
@@ -140,7 +147,7 @@ __global__ __launch_bounds__(64) void k_ccs_bwd(int n, const int* __restrict__ c
   const int i = n - 1 - sh;
   if (i < 0) return;
   const int p0 = cp[i], p1 = cp[i + 1];
-  const double acc = wave_sum(dot_awaited(p0 + 1, p1, lane, ri, v, x, err));
+  const double acc = wave_sum(dot_awaited<true>(p0 + 1, p1, lane, ri, v, x, err));
   if (lane == 0) publish(x + i, (y[i] - acc) / v[p0]);
 }
 
