@@ -244,13 +244,25 @@ void oracle_etree(int n, const int* colptr, const int* rowind, int* parent)
 
 /* ---- symbolic elimination ------------------------------------------------- */
 
+typedef struct { double zeros, nonzeros; } znz;
+
 typedef struct {
   int n, n_sn, depth;
   int *sn_size, *sn_up_size, *first_child, *next_child;
   int** sn_struct;
   double fund_nnz, fund_flops, relax_nnz, relax_flops;
   int failed;
+  /* merge log: one record per supernode with children that amalgamation
+     visits, in visit order: sn, its own (fundamental) size, the candidate
+     merged size and update size, the decision, and the candidate columns */
+  int nlog, caplog, ncols, capcols;
+  int* log_i;     /* 6 per record: sn, own_size, new_size, new_up, merged, cols_off */
+  double* log_d;  /* 2 per record: zeros, nonzeros */
+  int* log_cols;
 } oracle_sym;
+
+static void log_decision(oracle_sym* L, int sn, int new_size, int new_up, znz m, int merged,
+                         const int* cols);
 
 typedef struct {
   const int *colptr, *rowind;
@@ -326,8 +338,6 @@ static int rec_symbolic(sym_ctx* c, int j)
   return 0;
 }
 
-typedef struct { double zeros, nonzeros; } znz;
-
 static znz rec_amalgamate(sym_ctx* c, int sn)
 {
   oracle_sym* L = c->L;
@@ -386,12 +396,15 @@ static znz rec_amalgamate(sym_ctx* c, int sn)
   for (ip = 0; ip < new_up; ip++) merged.zeros += zcount[c->rowind_buf[ip]];
   free(zcount);
 
-  if (!((new_size < 16)
-        || ((L->sn_size[sn] < 50) && (merged.zeros < 0.5 * merged.nonzeros))
-        || ((L->sn_size[sn] < 250) && (merged.zeros < 0.25 * merged.nonzeros))
-        || ((L->sn_size[sn] < 500) && (merged.zeros < 0.10 * merged.nonzeros))
-        || (merged.zeros < 0.05 * merged.nonzeros)))
-    return own;
+  {
+    int take = (new_size < 16)
+               || ((L->sn_size[sn] < 50) && (merged.zeros < 0.5 * merged.nonzeros))
+               || ((L->sn_size[sn] < 250) && (merged.zeros < 0.25 * merged.nonzeros))
+               || ((L->sn_size[sn] < 500) && (merged.zeros < 0.10 * merged.nonzeros))
+               || (merged.zeros < 0.05 * merged.nonzeros);
+    log_decision(L, sn, new_size, new_up, merged, take, c->rowind_buf);
+    if (!take) return own;
+  }
 
   L->sn_size[sn] = new_size;
   L->sn_up_size[sn] = new_up;
@@ -417,6 +430,31 @@ static znz rec_amalgamate(sym_ctx* c, int sn)
   return merged;
 }
 
+static void log_decision(oracle_sym* L, int sn, int new_size, int new_up, znz m, int merged,
+                         const int* cols)
+{
+  if (L->caplog < 0) return;  /* logging off */
+  if (L->nlog == L->caplog) {
+    L->caplog = L->caplog ? 2 * L->caplog : 64;
+    L->log_i = realloc(L->log_i, sizeof(int) * 6 * L->caplog);
+    L->log_d = realloc(L->log_d, sizeof(double) * 2 * L->caplog);
+  }
+  if (L->ncols + new_size > L->capcols) {
+    while (L->ncols + new_size > L->capcols) L->capcols = L->capcols ? 2 * L->capcols : 1024;
+    L->log_cols = realloc(L->log_cols, sizeof(int) * L->capcols);
+  }
+  {
+    int* r = L->log_i + 6 * L->nlog;
+    r[0] = sn; r[1] = L->sn_size[sn]; r[2] = new_size; r[3] = new_up; r[4] = merged;
+    r[5] = L->ncols;
+  }
+  L->log_d[2 * L->nlog] = m.zeros;
+  L->log_d[2 * L->nlog + 1] = m.nonzeros;
+  memcpy(L->log_cols + L->ncols, cols, sizeof(int) * new_size);
+  L->ncols += new_size;
+  L->nlog++;
+}
+
 static void stats(oracle_sym* L, double* nnz, double* flops)
 {
   int sn, i, colnnz;
@@ -433,6 +471,8 @@ typedef struct {
   int n;
   const int *colptr, *rowind;
   int max_depth;
+  int amalgamate;  /* 0: stop after the fundamental supernodes */
+  int log_merges;
   oracle_sym* out;
 } sym_job;
 
@@ -445,6 +485,7 @@ static void* sym_thread(void* arg)
   int* parent = malloc(sizeof(int) * (n + 1));
   job->out = L;
   L->n = n;
+  L->caplog = job->log_merges ? 0 : -1;
   L->sn_struct = calloc(n + 1, sizeof(int*));
   L->sn_size = malloc(sizeof(int) * (n + 1));
   L->sn_up_size = malloc(sizeof(int) * (n + 1));
@@ -493,7 +534,7 @@ static void* sym_thread(void* arg)
     rec_symbolic(&c, n);
     stats(L, &L->fund_nnz, &L->fund_flops);
     for (j = 0; j < n; j++) c.map[j] = -1;
-    if (L->n_sn > 0) rec_amalgamate(&c, L->n_sn - 1);
+    if (L->n_sn > 0 && job->amalgamate) rec_amalgamate(&c, L->n_sn - 1);
     stats(L, &L->relax_nnz, &L->relax_flops);
   }
   free(parent); free(c.first_child); free(c.next_child); free(c.col2sn); free(c.map);
@@ -501,17 +542,35 @@ static void* sym_thread(void* arg)
   return NULL;
 }
 
-oracle_sym* oracle_symbolic(int n, const int* colptr, const int* rowind, int max_depth)
+/* flags: 1 = skip amalgamation (fundamental supernodes only), 2 = log the
+   amalgamation decisions */
+oracle_sym* oracle_symbolic_ex(int n, const int* colptr, const int* rowind, int max_depth, int flags)
 {
   pthread_t th;
   pthread_attr_t at;
-  sym_job job = {n, colptr, rowind, max_depth, NULL};
+  sym_job job = {n, colptr, rowind, max_depth, !(flags & 1), (flags & 2) != 0, NULL};
   pthread_attr_init(&at);
   pthread_attr_setstacksize(&at, (size_t)1 << 30);
   if (pthread_create(&th, &at, sym_thread, &job) != 0) return NULL;
   pthread_join(th, NULL);
   pthread_attr_destroy(&at);
   return job.out;
+}
+
+oracle_sym* oracle_symbolic(int n, const int* colptr, const int* rowind, int max_depth)
+{
+  return oracle_symbolic_ex(n, colptr, rowind, max_depth, 0);
+}
+
+/* The merge log (see oracle_sym).  Returns the record count; with non-NULL
+   arrays copies 6 ints and 2 doubles per record and the candidate columns. */
+int oracle_sym_merge_log(const oracle_sym* L, int* rec_i, double* rec_d, int* cols, int* ncols)
+{
+  if (rec_i && L->nlog) memcpy(rec_i, L->log_i, sizeof(int) * 6 * L->nlog);
+  if (rec_d && L->nlog) memcpy(rec_d, L->log_d, sizeof(double) * 2 * L->nlog);
+  if (cols && L->ncols) memcpy(cols, L->log_cols, sizeof(int) * L->ncols);
+  if (ncols) *ncols = L->ncols;
+  return L->nlog;
 }
 
 int oracle_sym_info(const oracle_sym* L, int* n_sn, int* depth, int* failed, double* st4)
@@ -549,7 +608,7 @@ void oracle_sym_free(oracle_sym* L)
   if (!L) return;
   for (sn = 0; sn < L->n; sn++) free(L->sn_struct[sn]);
   free(L->sn_struct); free(L->sn_size); free(L->sn_up_size); free(L->first_child);
-  free(L->next_child); free(L);
+  free(L->next_child); free(L->log_i); free(L->log_d); free(L->log_cols); free(L);
 }
 
 void oracle_free(void* p) { free(p); }

@@ -18,6 +18,7 @@ uniqueness of the Cholesky factor (checked against numpy.linalg.cholesky on
 small matrices in tests/test_oracle.py).
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -35,6 +36,9 @@ def lib():
         _lib = C.CDLL(path)
         _lib.oracle_symbolic.restype = C.c_void_p
         _lib.oracle_symbolic.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        _lib.oracle_symbolic_ex.restype = C.c_void_p
+        _lib.oracle_symbolic_ex.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        _lib.oracle_sym_merge_log.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         _lib.oracle_sym_info.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         _lib.oracle_sym_export.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         _lib.oracle_sym_free.argtypes = [C.c_void_p]
@@ -92,6 +96,78 @@ def mesh3d(X, Y, Z):
     return (X * Y * Z,) + _take(cp, ri, v, X * Y * Z, nnz)
 
 
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix_u01(count, seed):
+    """The seeded uniform[0,1) stream of the workloads' generators: splitmix64
+    (state += 0x9E3779B97F4A7C15, then the xor-shift-multiply finalizer), top
+    53 bits scaled by 2^-53.  Vectorized over the stream position; numpy's
+    uint64 arithmetic wraps modulo 2^64 like the C code's."""
+    i = np.arange(1, count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + i * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def uniform(count, seed, lo=-1.0, hi=1.0):
+    return lo + (hi - lo) * splitmix_u01(count, seed)
+
+
+def mesh3d_varcoef(X, Y, Z, seed, log10_lo=-2.0, log10_hi=2.0, shift=1e-6):
+    """Variable-coefficient 7-point operator on the mesh3d pattern and entry
+    order (the seeded workload generator, not a reference routine): every grid
+    edge draws c = 10^(lo + (hi-lo) u) from one stream in the generator's visit
+    order (z, y, x loops; x+1, y+1, z+1 edges of each point); off-diagonal -c;
+    diagonal = incident c summed in the order +x, +y, +z, -x, -y, -z, then the
+    shift."""
+    N = X * Y * Z
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    x, y, z = x.ravel(), y.ravel(), z.ravel()
+    has = np.stack([x < X - 1, y < Y - 1, z < Z - 1], 1)        # per point: x+1, y+1, z+1
+    u = splitmix_u01(int(has.sum()), seed)
+    w = np.zeros((N, 3))
+    expo = log10_lo + (log10_hi - log10_lo) * u
+    # C's pow (numpy's vectorized power may differ from it in the last bit)
+    w[has] = np.fromiter((math.pow(10.0, e) for e in expo), np.float64, len(expo))
+    wx, wy, wz = w[:, 0], w[:, 1], w[:, 2]
+    d = np.zeros(N)
+    for term in (wx, wy, wz):
+        d = d + term
+    j = np.arange(N)
+    for sel, off, ww in ((x > 0, 1, wx), (y > 0, X, wy), (z > 0, X * Y, wz)):
+        t = np.zeros(N)
+        t[sel] = ww[j[sel] - off]
+        d = d + t
+    d = d + shift
+    cnt = has.sum(1) + 1
+    cp = np.zeros(N + 1, np.int64)
+    cp[1:] = np.cumsum(cnt)
+    ri = np.empty(cp[-1], np.int32)
+    v = np.empty(cp[-1])
+    pos = cp[:-1].copy()
+    for k, off in enumerate((1, X, X * Y)):
+        sel = has[:, k]
+        ri[pos[sel]] = j[sel] + off
+        v[pos[sel]] = -w[sel, k]
+        pos[sel] += 1
+    ri[pos] = j
+    v[pos] = d
+    return N, cp.astype(np.int32), ri, v
+
+
+def mesh2d_shifted(s, seed):
+    """mesh2d(s, "neumann") plus an i.i.d. uniform[0,1) shift on each diagonal
+    entry (the last entry of each column), drawn in column order."""
+    n, cp, ri, v = mesh2d(s)
+    v = v.copy()
+    v[cp[1:] - 1] += splitmix_u01(n, seed)
+    return n, cp, ri, v
+
+
 def genmmd(n, colptr, rowind):
     _load_ref()
     colptr = np.ascontiguousarray(colptr, np.int32)
@@ -122,11 +198,17 @@ def etree(n, colptr, rowind):
     return parent
 
 
-def symbolic(n, colptr, rowind, max_depth=0):
-    """Supernodal structure exactly as taucs_ccs_symbolic_elimination leaves it."""
+def symbolic(n, colptr, rowind, max_depth=0, amalgamate=True, merge_log=False):
+    """Supernodal structure exactly as taucs_ccs_symbolic_elimination leaves it.
+    amalgamate=False stops after the fundamental supernodes
+    (recursive_symbolic_elimination alone); merge_log=True also returns, per
+    supernode with children that recursive_amalgamate_supernodes visits, the
+    decision it took: (sn, own size, candidate size, candidate update size,
+    merged, zeros, nonzeros, candidate columns)."""
     colptr = np.ascontiguousarray(colptr, np.int32)
     rowind = np.ascontiguousarray(rowind, np.int32)
-    h = lib().oracle_symbolic(n, _p(colptr), _p(rowind), max_depth)
+    flags = (0 if amalgamate else 1) | (2 if merge_log else 0)
+    h = lib().oracle_symbolic_ex(n, _p(colptr), _p(rowind), max_depth, flags)
     if not h:
         raise RuntimeError("oracle symbolic failed")
     ns, dp, fl = C.c_int(), C.c_int(), C.c_int()
@@ -142,6 +224,17 @@ def symbolic(n, colptr, rowind, max_depth=0):
         lib().oracle_sym_export(h, None, None, None, None, None, _p(idx))
         out.update(sn_size=sz[:ns], sn_up_size=up[:ns], first_child=fc, next_child=nc,
                    struct_ptr=sp_, struct_idx=idx[:tot])
+    if merge_log:
+        nc_ = C.c_int()
+        cnt = lib().oracle_sym_merge_log(h, None, None, None, C.byref(nc_))
+        ri_ = np.zeros((max(cnt, 1), 6), np.int32)
+        rd_ = np.zeros((max(cnt, 1), 2))
+        cols = np.zeros(max(nc_.value, 1), np.int32)
+        lib().oracle_sym_merge_log(h, _p(ri_), _p(rd_), _p(cols), None)
+        out["merge_log"] = [dict(sn=int(r[0]), own_size=int(r[1]), new_size=int(r[2]),
+                                 new_up=int(r[3]), merged=bool(r[4]), zeros=float(d[0]),
+                                 nonzeros=float(d[1]), cols=cols[r[5]:r[5] + r[2]].copy())
+                            for r, d in zip(ri_[:cnt], rd_[:cnt])]
     lib().oracle_sym_free(h)
     return out
 

@@ -349,3 +349,17 @@ def test_ccs_solve_llt_rejects_non_factors():
     for M in cases:
         x = np.zeros(M.n)
         assert T.lib.taucs_ccs_solve_llt(M.ptr, T._dptr(x), T._dptr(np.ones(M.n))) == -1
+
+
+@pytest.mark.parametrize("name", ["comb20", "two_children", "star5", "path5", "mesh2d_3"])
+def test_product_supernodes_equal_the_hand_derived_ones(name):
+    """The product's symbolic phase on the small patterns whose supernodes,
+    row order and child lists are derived by hand from the reference's rules
+    (tests/test_oracle_order.py holds the derivations)."""
+    from tests.test_oracle_order import EXAMPLES, ccs, structure
+    n, cols, _, final = EXAMPLES[name]()
+    cp, ri, v = ccs(n, cols)
+    S = T.factor_symbolic(T.from_arrays(n, cp, ri, v)).supernodes()
+    n_sn, live, kids = final
+    got_live, got_kids = structure(S)
+    assert S["n_sn"] == n_sn and got_live == live and got_kids == kids
