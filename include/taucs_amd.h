@@ -62,6 +62,11 @@ int taucs_amd_factor_get_supernodes(void* L, int* sn_size, int* sn_up_size,
 int taucs_amd_factor_get_schedule(void* L, int* sn_level, int* sn_parent, long long* off_L,
                                   long long* off_U);
 
+/* Digest of the numeric factor's stored bits (a position-mixed sum modulo
+   2^64 over the device L arena): two factors of one symbolic structure are
+   bitwise equal iff their digests are (up to 2^-64). */
+int taucs_amd_factor_digest(void* L, unsigned long long* digest);
+
 /* ---- solves ------------------------------------------------------------- */
 /* X, B column-major host arrays (n x nrhs); permuted space like
    taucs_supernodal_solve_llt. */
@@ -88,12 +93,27 @@ typedef struct {
   double rel_residual;   /* max over rhs of |b-Ax|_2 / |b|_2 */
 } taucs_amd_refine_info;
 
-/* Solve A X = B (A = the permuted matrix the factor was built from) with
-   FP64 residuals on the device and the factor as preconditioner.
-   method: 0 = classical refinement, 1 = preconditioned conjugate gradients. */
+/* Solve A X = B (A in the permuted space of the factor; its pattern must be
+   the factored one, its values may differ) with FP64 residuals on the device
+   and the factor as preconditioner.
+   method: 0 = classical refinement, 1 = preconditioned conjugate gradients,
+   ORed with the flags below.  By default each right-hand side stops once its
+   normwise backward error |b-Ax|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol.
+   A's values are uploaded on every call (A may have changed in place since
+   the factorization) unless FACTORED_VALUES is given. */
+#define TAUCS_AMD_REFINE_FACTORED_VALUES 0x100 /* A's values are exactly those of the
+                                                  factor's last numeric factorization:
+                                                  use the factor's device copy */
+#define TAUCS_AMD_REFINE_STOP_RELRES     0x200 /* stop on |b-Ax|_2 <= tol |b|_2, the rule
+                                                  of the reference's conjugate gradients
+                                                  (src/taucs_iter.c:139,186) */
 int taucs_amd_solve_refined(void* L, taucs_ccs_matrix* A, int nrhs, double* X, int ldx,
                             const double* B, int ldb, int method, double tol, int maxit,
                             taucs_amd_refine_info* info);
+/* Same, with X and B in device memory. */
+int taucs_amd_solve_refined_device(void* L, taucs_ccs_matrix* A, int nrhs, double* dX, int ldx,
+                                   const double* dB, int ldb, int method, double tol, int maxit,
+                                   taucs_amd_refine_info* info);
 
 /* ---- many independent systems in the same launches ---------------------- */
 /* A[q] are the permuted matrices P A P^T of `count` independent systems.  Each

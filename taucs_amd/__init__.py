@@ -105,6 +105,10 @@ _sigs = {
     "taucs_amd_solve_refined": (C.c_int, [C.c_void_p, _ccs_p, C.c_int, C.c_void_p, C.c_int,
                                           C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int,
                                           P(RefineInfo)]),
+    "taucs_amd_factor_digest": (C.c_int, [C.c_void_p, P(C.c_uint64)]),
+    "taucs_amd_solve_refined_device": (C.c_int, [C.c_void_p, _ccs_p, C.c_int, C.c_void_p, C.c_int,
+                                                 C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int,
+                                                 P(RefineInfo)]),
     "taucs_amd_set_profiling": (None, [C.c_int]),
     "taucs_amd_set_graphs": (None, [C.c_int]),
     "taucs_amd_set_lookahead": (None, [C.c_int]),
@@ -312,6 +316,15 @@ def etree(A):
     return parent
 
 
+REFINE_FACTORED_VALUES = 0x100
+REFINE_STOP_RELRES = 0x200
+
+
+def _refine_method(method, factored, stop_relres):
+    return int(method) | (REFINE_FACTORED_VALUES if factored else 0) | \
+        (REFINE_STOP_RELRES if stop_relres else 0)
+
+
 class Factor:
     """Owner of an opaque device-resident supernodal factor."""
 
@@ -367,18 +380,42 @@ class Factor:
             raise RuntimeError("solve failed")
         return X[:, 0].copy() if one else X
 
-    def solve_refined(self, A, b, method=0, tol=1e-14, maxit=100):
+    def solve_refined(self, A, b, method=0, tol=1e-14, maxit=100, factored=False,
+                      stop_relres=False):
+        """FP64 refinement (method 0) or preconditioned CG (method 1) on A.
+        factored=True: A's values are those this factor was last factored from
+        (the factor's device copy is used instead of uploading A's values).
+        stop_relres=True: stop on |r|_2 <= tol |b|_2 instead of the backward error."""
         b = np.asarray(b, dtype=np.float64)
         one = b.ndim == 1
         B = np.asfortranarray(b.reshape(b.shape[0], -1))
         X = np.empty_like(B, order="F")
         n, k = B.shape
         info = RefineInfo()
-        rc = lib.taucs_amd_solve_refined(self.h, A.ptr, k, _dptr(X), n, _dptr(B), n, method, tol,
+        m = _refine_method(method, factored, stop_relres)
+        rc = lib.taucs_amd_solve_refined(self.h, A.ptr, k, _dptr(X), n, _dptr(B), n, m, tol,
                                          maxit, C.byref(info))
         if rc != 0:
             raise RuntimeError("refined solve failed")
         return (X[:, 0].copy() if one else X), info
+
+    def solve_refined_device(self, A, dX, dB, n, nrhs, method=0, tol=1e-14, maxit=100,
+                             factored=False, stop_relres=False):
+        """solve_refined with right-hand sides and solutions in HBM."""
+        info = RefineInfo()
+        m = _refine_method(method, factored, stop_relres)
+        rc = lib.taucs_amd_solve_refined_device(self.h, A.ptr, nrhs, dX, n, dB, n, m, tol, maxit,
+                                                C.byref(info))
+        if rc != 0:
+            raise RuntimeError("refined solve failed")
+        return info
+
+    def digest(self):
+        """Digest of the numeric factor's stored bits (equal iff bitwise equal)."""
+        d = C.c_uint64()
+        if lib.taucs_amd_factor_digest(self.h, C.byref(d)) != 0:
+            raise RuntimeError("factor digest failed")
+        return d.value
 
     def device_bytes(self, precision=FP64, nrhs=1, refine=False):
         """Device bytes this factor holds once factored and solved (host estimate)."""

@@ -45,7 +45,8 @@ __device__ __forceinline__ double wave_sum(double v)
 // The wait of one value, bounded by a larger limit than the supernodal
 // sweeps use: on a large factor a row can wait for a chain of thousands of
 // long rows (the root separator's), while the other waiting rows poll.
-// Tickets guarantee progress; the bound only turns a fault into an error.
+// Tickets guarantee progress; the bound only turns a fault into an error, and
+// once one wait has marked the error word the others stop at their next poll.
 #define CCS_SPIN_LIMIT (1 << 26)
 __device__ __forceinline__ double await_long(const double* p, int* err)
 {
@@ -57,6 +58,8 @@ __device__ __forceinline__ double await_long(const double* p, int* err)
       atomicOr(err, 1);
       break;
     }
+    // another wait of this sweep already gave up: stop polling too
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
     __builtin_amdgcn_s_sleep(2);
     v = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }

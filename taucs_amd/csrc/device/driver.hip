@@ -1406,19 +1406,29 @@ extern "C" int taucs_amd_get_numeric_timing(void* vL, taucs_amd_numeric_timing* 
 
 // ---- solves ------------------------------------------------------------------
 
+// The workspace of one sweep: front blocks W (W_per_rhs x nrhs), the forward
+// solution Y (n x nrhs), split-K partial blocks, then the chained launches'
+// tickets followed by the error word.
+struct SolveBufs {
+  void* W;
+  void* Y;
+  void* Part;
+  int* sync;
+};
+
 template <typename T>
-static int emit_solve(tamd_dev* D, Emitter& E, int n, int nrhs, double* dX, int ldx,
-                      const double* dB, int ldb)
+static int emit_solve(tamd_dev* D, Emitter& E, const SolveBufs& sb, int n, int nrhs, double* dX,
+                      int ldx, const double* dB, int ldb)
 {
-  T* W = (T*)D->W;
-  T* Y = (T*)D->Y;
-  T* Part = (T*)D->Part;
+  T* W = (T*)sb.W;
+  T* Y = (T*)sb.Y;
+  T* Part = (T*)sb.Part;
   const T* L = (const T*)D->L;
   const T* Iv = (const T*)D->Inv;
   const FrontDesc* fr = D->d_fronts;
   const int* rows = D->d_rows;
-  int* tickets = D->d_sync;
-  int* err = D->d_sync + D->sync_n;
+  int* tickets = sb.sync;
+  int* err = sb.sync + D->sync_n;
   for (const Op& op : D->solve_chain ? D->sv_chain_ops : D->sv_ops) {
     const PanelTask* tk = D->d_sv_tasks + op.start;
     const dim3 grid(op.count);
@@ -1517,9 +1527,10 @@ static int run_solve(tamd_factor* F, int nrhs, double* dX, int ldx, const double
     if (dev_malloc((void**)&D->d_sync, sizeof(int) * (size_t)(D->sync_n + 1)) != 0) return -1;
     HIPCHECK(hipMemsetAsync(D->d_sync, 0, sizeof(int) * (size_t)(D->sync_n + 1), st));
   }
+  const SolveBufs sb = {D->W, D->Y, D->Part, D->d_sync};
   if (!g_graphs) {
     Emitter E = eager_emitter(D);
-    if (emit_solve<T>(D, E, n, nrhs, dX, ldx, dB, ldb) != 0) return -1;
+    if (emit_solve<T>(D, E, sb, n, nrhs, dX, ldx, dB, ldb) != 0) return -1;
     HIPCHECK(hipGetLastError());
     return 0;
   }
@@ -1538,7 +1549,7 @@ static int run_solve(tamd_factor* F, int nrhs, double* dX, int ldx, const double
   Emitter E;
   E.main_st = st;
   HIPCHECK(hipGraphCreate(&E.g, 0));
-  int rc = emit_solve<T>(D, E, n, nrhs, dX, ldx, dB, ldb);
+  int rc = emit_solve<T>(D, E, sb, n, nrhs, dX, ldx, dB, ldb);
   hipError_t ie = rc == 0 ? hipGraphInstantiate(&slot->exec, E.g, nullptr, nullptr, 0)
                           : hipErrorUnknown;
   (void)hipGraphDestroy(E.g);
@@ -1612,6 +1623,36 @@ extern "C" int taucs_amd_supernodal_solve_many_device(void* vL, int nrhs, double
   return tamd_dev_sweep_error(F);
 }
 
+// Internal hooks of the allocation-free solve (noalloc.hip): the workspace
+// dimensions of one sweep, and one sweep launched eagerly on the factor's
+// stream with workspace the caller provides.
+extern "C" int tamd_dev_solve_dims(tamd_factor* F, long long* W_per_rhs, long long* part_slots,
+                                   long long* sync_n, int* elem)
+{
+  tamd_dev* D = F ? F->dev : nullptr;
+  if (!D) return -1;
+  *W_per_rhs = D->W_per_rhs;
+  *part_slots = std::max(D->part_slots, 1LL);
+  *sync_n = D->sync_n;
+  *elem = D->elem;
+  return 0;
+}
+
+extern "C" int tamd_dev_solve_eager(tamd_factor* F, void* W, void* Y, void* Part, int* sync, int nr,
+                                    double* dX, int ldx, const double* dB, int ldb)
+{
+  tamd_dev* D = F ? F->dev : nullptr;
+  if (!D || !D->L || nr <= 0 || nr > 16) return -1;
+  const SolveBufs sb = {W, Y, Part, sync};
+  Emitter E = eager_emitter(D);
+  const int rc = D->prec == TAMD_PREC_FP64
+                     ? emit_solve<double>(D, E, sb, F->n, nr, dX, ldx, dB, ldb)
+                     : emit_solve<float>(D, E, sb, F->n, nr, dX, ldx, dB, ldb);
+  if (rc != 0) return -1;
+  HIPCHECK(hipGetLastError());
+  return 0;
+}
+
 extern "C" int tamd_dev_solve(tamd_factor* F, int nrhs, double* X, int ldx, const double* B, int ldb)
 {
   tamd_dev* D = F->dev;
@@ -1650,6 +1691,49 @@ extern "C" int tamd_dev_download_blocks(tamd_factor* F, double* Lhost)
     for (size_t i = 0; i < n; i++) Lhost[i] = tmp[i];
   }
   return 0;
+}
+
+// Digest of the factor's stored bits: the sum modulo 2^64 of a mix of every
+// 32-bit word of the L arena with its position.  Integer sums do not depend
+// on the reduction order, so equal factors give equal digests; a change of
+// any bit changes the digest (up to 2^-64).  Used to compare factors of
+// different schedules bit for bit without copying them out.
+__global__ __launch_bounds__(256) void k_digest(const unsigned int* __restrict__ w, long long nw,
+                                                unsigned long long* out)
+{
+  unsigned long long acc = 0;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < nw; i += (long long)gridDim.x * 256) {
+    unsigned long long z = ((unsigned long long)(i + 1) << 32 | w[i]) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    acc += z ^ (z >> 31);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+extern "C" int taucs_amd_factor_digest(void* vL, unsigned long long* digest)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  tamd_dev* D = F ? F->dev : nullptr;
+  if (!D || !D->L || !F->has_numeric || !digest) return -1;
+  unsigned long long* d_out = nullptr;
+  if (dev_malloc((void**)&d_out, sizeof(unsigned long long)) != 0) return -1;
+  const long long nw = D->L_elems * (long long)D->elem / 4;
+  int rc = 0;
+  if (hipMemsetAsync(d_out, 0, sizeof(unsigned long long), D->stream) != hipSuccess) rc = -1;
+  if (!rc) {
+    hipLaunchKernelGGL(k_digest, dim3(2048), dim3(256), 0, D->stream, (const unsigned int*)D->L, nw,
+                       d_out);
+    if (hipGetLastError() != hipSuccess
+        || hipMemcpyAsync(digest, d_out, sizeof(unsigned long long), hipMemcpyDeviceToHost, D->stream)
+               != hipSuccess
+        || hipStreamSynchronize(D->stream) != hipSuccess)
+      rc = -1;
+  }
+  tamd_dev_free_ptr(d_out);
+  return rc;
 }
 
 // ---- timing for the schedule autotuner ---------------------------------------

@@ -114,7 +114,8 @@ __device__ __forceinline__ void publish(T* p, T v)
 }
 
 // one value, awaited (bounded: after SPIN_LIMIT polls the error word is marked
-// and the pattern is returned as it is)
+// and the pattern is returned as it is; once any wait of the launch has marked
+// the error word, every other wait gives up at its next poll)
 template <typename T>
 __device__ __forceinline__ T await_value(const T* p, int* err)
 {
@@ -127,6 +128,7 @@ __device__ __forceinline__ T await_value(const T* p, int* err)
       atomicOr(err, 1);
       break;
     }
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
     __builtin_amdgcn_s_sleep(1);
     v = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }

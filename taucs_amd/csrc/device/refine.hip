@@ -31,6 +31,8 @@
 namespace tamd {
 
 // y(:,r) = b(:,r) - A x(:,r)     (b may be null: y = A x)
+// One thread per row; A's row is read once per group of four right-hand
+// sides, and each sum runs over the row in stored order.
 __global__ void k_csr_residual(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                const double* __restrict__ v, const double* __restrict__ x,
                                int ldx, const double* __restrict__ b, int ldb,
@@ -38,10 +40,24 @@ __global__ void k_csr_residual(int n, const int* __restrict__ rp, const int* __r
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  for (int r = 0; r < nrhs; r++) {
-    double acc = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; k++) acc += v[k] * x[ci[k] + (long long)r * ldx];
-    y[i + (long long)r * ldy] = b ? b[i + (long long)r * ldb] - acc : acc;
+  const int k0 = rp[i], k1 = rp[i + 1];
+  for (int r0 = 0; r0 < nrhs; r0 += 4) {
+    const int nr = nrhs - r0 < 4 ? nrhs - r0 : 4;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double* xr = x + (long long)r0 * ldx;
+    for (int k = k0; k < k1; k++) {
+      const double a = v[k];
+      const long long c = ci[k];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (q < nr) acc[q] += a * xr[c + (long long)q * ldx];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (q < nr) {
+        const long long r = r0 + q;
+        y[i + r * ldy] = b ? b[i + r * ldb] - acc[q] : acc[q];
+      }
   }
 }
 
@@ -128,9 +144,8 @@ __global__ void k_ratio(const double* num, const double* den, const int* done, d
 using namespace tamd;
 
 struct RefineCtx {
-  // pattern identity (the CSR structure depends only on it)
-  const int* colptr = nullptr;
-  const int* rowind = nullptr;
+  // the pattern the CSR structure was built from (compared by content)
+  std::vector<int> pat_cp, pat_ri;
   int n = 0, nnz = 0;      // nnz: stored (lower) entries of A
   int* rp = nullptr;
   int* ci = nullptr;
@@ -199,7 +214,8 @@ __global__ void k_gather_vals(int nnz, const int* __restrict__ src, const double
 
 // The full-pattern CSR of A (built once per pattern) and its values (gathered
 // on the device whenever the factor was refactored or A's values changed).
-static RefineCtx* get_ctx(tamd_factor* F, const taucs_ccs_matrix* A, int nrhs, hipStream_t st)
+static RefineCtx* get_ctx(tamd_factor* F, const taucs_ccs_matrix* A, int nrhs, bool factored_values,
+                          hipStream_t st)
 {
   void* key = F;
   RefineCtx* c = nullptr;
@@ -208,8 +224,9 @@ static RefineCtx* get_ctx(tamd_factor* F, const taucs_ccs_matrix* A, int nrhs, h
     for (auto& e : g_ctx)
       if (e.key == key) c = e.ctx;
   }
-  if (c && (c->colptr != A->colptr || c->rowind != A->rowind || c->n != A->n
-            || c->nnz != A->colptr[A->n])) {
+  if (c && (c->n != A->n || c->nnz != A->colptr[A->n]
+            || memcmp(c->pat_cp.data(), A->colptr, sizeof(int) * (size_t)(A->n + 1)) != 0
+            || memcmp(c->pat_ri.data(), A->rowind, sizeof(int) * (size_t)c->nnz) != 0)) {
     tamd_refine_forget(key);
     c = nullptr;
   }
@@ -232,7 +249,9 @@ static RefineCtx* get_ctx(tamd_factor* F, const taucs_ccs_matrix* A, int nrhs, h
         ci[fill[i]] = j; src[fill[i]++] = k;
         if (i != j) { ci[fill[j]] = i; src[fill[j]++] = k; }
       }
-    c->colptr = A->colptr; c->rowind = A->rowind; c->n = n; c->nnz = nl;
+    c->pat_cp.assign(A->colptr, A->colptr + n + 1);
+    c->pat_ri.assign(A->rowind, A->rowind + nl);
+    c->n = n; c->nnz = nl;
     c->nfull = nnz;
     if (dmalloc((void**)&c->rp, sizeof(int) * (n + 1)) || dmalloc((void**)&c->ci, sizeof(int) * nnz)
         || dmalloc((void**)&c->src, sizeof(int) * nnz) || dmalloc((void**)&c->v, sizeof(double) * nnz)) {
@@ -249,14 +268,16 @@ static RefineCtx* get_ctx(tamd_factor* F, const taucs_ccs_matrix* A, int nrhs, h
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     g_ctx.push_back({key, c});
   }
-  // values: the factor's own device copy when A is the matrix it was factored
-  // from, else A's host values uploaded now
+  // values: the factor's own device copy when the caller vouches that A's
+  // values are those of the last numeric factorization, else A's host values
+  // uploaded on every call (A may have changed in place, or be a new matrix at
+  // a recycled address)
   const double* dvals = nullptr;
   const double* hsrc = nullptr;
   unsigned long long gen = 0;
   if (tamd_dev_values_of(F, &dvals, &hsrc, &gen) != 0) return nullptr;
   const double* from;
-  if (hsrc == A->values.d) {
+  if (factored_values) {
     from = dvals;
   } else {
     if (!c->avals && dmalloc((void**)&c->avals, sizeof(double) * (size_t)(c->nnz > 0 ? c->nnz : 1)))
@@ -343,16 +364,14 @@ static int residual_and_error(hipStream_t st, RefineCtx* c, const double* x, con
   return 0;
 }
 
-extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, double* X, int ldx,
-                                       const double* B, int ldb, int method, double tol, int maxit,
-                                       taucs_amd_refine_info* info)
+// The refinement loop on device vectors: c->b holds B, the result is left in
+// c->x.  Flags (in `method`): TAUCS_AMD_REFINE_FACTORED_VALUES,
+// TAUCS_AMD_REFINE_STOP_RELRES.
+static int refine_core(tamd_factor* F, RefineCtx* c, hipStream_t st, int nrhs, int method, double tol,
+                       int maxit, taucs_amd_refine_info* info)
 {
-  tamd_factor* F = (tamd_factor*)vL;
-  if (!F || !F->has_numeric || !A || A->n != F->n || nrhs <= 0) return -1;
-  hipStream_t st;
-  if (tamd_dev_stream_of(F, &st) != 0) return -1;
-  RefineCtx* c = get_ctx(F, A, nrhs, st);
-  if (!c) return -1;
+  const bool stop_relres = (method & TAUCS_AMD_REFINE_STOP_RELRES) != 0;
+  method &= 0xff;
   int n = c->n;
   if (c->anorm_stale) {  // |A|_inf = max row sum of |a_ij|, on the device
     hipLaunchKernelGGL(k_csr_absrow, dim3((n + 255) / 256), dim3(256), 0, st, n, c->rp, c->v, c->r);
@@ -362,8 +381,6 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
     c->anorm_stale = false;
   }
   if (maxit <= 0) maxit = 100;
-  HIPOK(hipMemcpy2DAsync(c->b, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
-                         hipMemcpyHostToDevice, st));
   std::vector<double> eta, relres;
   int it = 0;
   bool conv = false;
@@ -389,7 +406,7 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
       bool all = true, settled = true;
       for (int q = 0; q < nrhs; q++) {
         if (!done[q] && !bad[q] && diverging(q)) bad[q] = 1;
-        if (!bad[q] && eta[q] <= tol) done[q] = 1;
+        if (!bad[q] && (stop_relres ? relres[q] : eta[q]) <= tol) done[q] = 1;
         mask[q] = (done[q] || bad[q]) ? 0.0 : 1.0;
         all = all && done[q];
         settled = settled && (done[q] || bad[q]);
@@ -424,7 +441,7 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
       bool all = true, settled = true;
       for (int q = 0; q < nrhs; q++) {
         if (!done[q] && !bad[q] && diverging(q)) bad[q] = 1;
-        if (!bad[q] && eta[q] <= tol) done[q] = 1;
+        if (!bad[q] && (stop_relres ? relres[q] : eta[q]) <= tol) done[q] = 1;
         all = all && done[q];
         settled = settled && (done[q] || bad[q]);
         stop[q] = done[q] || bad[q];  // its recurrence stops (the device mask)
@@ -441,9 +458,6 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
     }
     if (residual_and_error(st, c, c->x, c->b, c->r, nrhs, eta, relres)) return -1;
   }
-  HIPOK(hipMemcpy2DAsync(X, sizeof(double) * ldx, c->x, sizeof(double) * n, sizeof(double) * n, nrhs,
-                         hipMemcpyDeviceToHost, st));
-  HIPOK(hipStreamSynchronize(st));
   if (tamd_dev_sweep_error(F)) return -1;
   if (info) {
     info->iterations = it;
@@ -451,5 +465,49 @@ extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, 
     info->backward_error = eta.empty() ? 0 : *std::max_element(eta.begin(), eta.end());
     info->rel_residual = relres.empty() ? 0 : *std::max_element(relres.begin(), relres.end());
   }
+  return 0;
+}
+
+static RefineCtx* refine_setup(tamd_factor* F, taucs_ccs_matrix* A, int nrhs, int method, hipStream_t* st)
+{
+  if (!F || !F->has_numeric || !A || A->n != F->n || nrhs <= 0) return nullptr;
+  if ((method & 0xff) > 1) return nullptr;
+  if (tamd_dev_stream_of(F, st) != 0) return nullptr;
+  return get_ctx(F, A, nrhs, (method & TAUCS_AMD_REFINE_FACTORED_VALUES) != 0, *st);
+}
+
+extern "C" int taucs_amd_solve_refined(void* vL, taucs_ccs_matrix* A, int nrhs, double* X, int ldx,
+                                       const double* B, int ldb, int method, double tol, int maxit,
+                                       taucs_amd_refine_info* info)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  hipStream_t st;
+  RefineCtx* c = refine_setup(F, A, nrhs, method, &st);
+  if (!c) return -1;
+  const int n = c->n;
+  HIPOK(hipMemcpy2DAsync(c->b, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                         hipMemcpyHostToDevice, st));
+  if (refine_core(F, c, st, nrhs, method, tol, maxit, info) != 0) return -1;
+  HIPOK(hipMemcpy2DAsync(X, sizeof(double) * ldx, c->x, sizeof(double) * n, sizeof(double) * n, nrhs,
+                         hipMemcpyDeviceToHost, st));
+  HIPOK(hipStreamSynchronize(st));
+  return 0;
+}
+
+extern "C" int taucs_amd_solve_refined_device(void* vL, taucs_ccs_matrix* A, int nrhs, double* dX,
+                                              int ldx, const double* dB, int ldb, int method, double tol,
+                                              int maxit, taucs_amd_refine_info* info)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  hipStream_t st;
+  RefineCtx* c = refine_setup(F, A, nrhs, method, &st);
+  if (!c) return -1;
+  const int n = c->n;
+  HIPOK(hipMemcpy2DAsync(c->b, sizeof(double) * n, dB, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                         hipMemcpyDeviceToDevice, st));
+  if (refine_core(F, c, st, nrhs, method, tol, maxit, info) != 0) return -1;
+  HIPOK(hipMemcpy2DAsync(dX, sizeof(double) * ldx, c->x, sizeof(double) * n, sizeof(double) * n, nrhs,
+                         hipMemcpyDeviceToDevice, st));
+  HIPOK(hipStreamSynchronize(st));
   return 0;
 }

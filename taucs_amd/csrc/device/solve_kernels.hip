@@ -420,7 +420,8 @@ __global__ __launch_bounds__(256) void k_bwd_step(const FrontDesc* __restrict__ 
 // task waits only for tasks of smaller index, and a task that holds a ticket
 // is running, so the smallest unfinished ticket can always proceed.  Every
 // wait is still bounded: after SPIN_LIMIT polls the value is taken as it is,
-// the error word is marked and the task finishes (the solve reports failure).
+// the error word is marked and the task finishes (the solve reports failure);
+// every other wait of the launch then gives up at its next poll.
 // Launches whose tasks wait for nothing run with ticket == nullptr (task =
 // blockIdx).
 
@@ -462,6 +463,7 @@ __device__ __forceinline__ void await_rows(T* Vl, const V* Vg, int ldv, int col,
           atomicOr(err, 1);
           break;
         }
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
         __builtin_amdgcn_s_sleep(1);
         bits[k] = __hip_atomic_load(adr[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }

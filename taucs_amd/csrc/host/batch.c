@@ -153,8 +153,10 @@ int taucs_amd_batch_solve_refined(void* vB, int nrhs, double* X, int ldx, const 
 {
   tamd_batch* B = (tamd_batch*)vB;
   if (!B || !B->all.has_numeric) return -1;
-  return taucs_amd_solve_refined(&B->all, B->blockdiag, nrhs, X, ldx, Bv, ldb, method, tol, maxit,
-                                 info);
+  /* the residuals are those of the systems as last factored (from host or
+     device values), read from the factor's own device copy */
+  return taucs_amd_solve_refined(&B->all, B->blockdiag, nrhs, X, ldx, Bv, ldb,
+                                 method | TAUCS_AMD_REFINE_FACTORED_VALUES, tol, maxit, info);
 }
 
 int taucs_amd_batch_info(void* vB, int* count, int* n_total, long long* xoff, double* fund_flops,

@@ -254,11 +254,25 @@ int taucs_linsolve(taucs_ccs_matrix* A, void** F, int nrhs, void* X, void* B, ch
                         f->rowperm);
     if (opt_cg || opt_refine || precision != TAMD_PREC_FP64) {
       taucs_amd_refine_info info;
+      int method;
+      double tol;
       if (!PAPT) PAPT = taucs_ccs_permute_symmetrically(A, f->rowperm, f->colperm);
       if (!PAPT) { rc = TAUCS_ERROR_NOMEM; goto fail_solve; }
-      if (taucs_amd_solve_refined(f->L, PAPT, nrhs, PX, n, PB, n, opt_cg ? 1 : 0,
-                                  opt_refine || precision != TAMD_PREC_FP64 ? 1e-14 : opt_tol,
-                                  (int)opt_maxits, &info) != 0)
+      if (opt_cg) {
+        /* conjugate gradients preconditioned by the factor, stopping like the
+           reference's on |r|_2 <= convergetol |b|_2 (src/taucs_iter.c:139,186;
+           x0 = 0, so r0 = b) */
+        method = 1 | TAUCS_AMD_REFINE_STOP_RELRES;
+        tol = opt_tol;
+      } else {
+        /* refinement to working accuracy in the normwise backward error */
+        method = 0;
+        tol = 1e-14;
+      }
+      /* the factor holds this PAPT's values only when this call factored it */
+      if (opt_factor) method |= TAUCS_AMD_REFINE_FACTORED_VALUES;
+      if (taucs_amd_solve_refined(f->L, PAPT, nrhs, PX, n, PB, n, method, tol, (int)opt_maxits,
+                                  &info) != 0)
         rc = TAUCS_ERROR;
     } else {
       if (taucs_amd_supernodal_solve_many(f->L, nrhs, PX, n, PB, n) != 0) rc = TAUCS_ERROR;

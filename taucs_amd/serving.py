@@ -290,7 +290,7 @@ class Server:
     def _solve(self, ent, reqs):
         cols = [r.B.shape[1] for r in reqs]
         Bp = np.asfortranarray(np.concatenate([r.B[ent.perm, :] for r in reqs], axis=1))
-        X, info = ent.F.solve_refined(ent.PA, Bp, method=self.method, tol=self.tol,
+        X, info = ent.F.solve_refined(ent.PA, Bp, method=self.method, tol=self.tol, factored=True,
                                       maxit=self.maxit)
         X = X.reshape(Bp.shape)
         # per-request backward error in the original ordering (host, FP64)

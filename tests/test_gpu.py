@@ -227,6 +227,69 @@ def test_linsolve_protocol():
     assert rc == T.TAUCS_SUCCESS and rres(Y) <= 1e-13
 
 
+def test_linsolve_cg_stops_on_the_reference_rule():
+    """taucs.solve.cg=true runs CG preconditioned by the factor and stops on
+    |r|_2 <= convergetol |b|_2 (src/taucs_iter.c:139,186), for the FP64 and a
+    reduced-precision factor."""
+    print("case: linsolve cg convergetol", flush=True)
+    A = T.mesh3d_varcoef(16, 16, 16, seed=11)
+    n = A.n
+    B = np.stack([T.random_uniform(n, 40 + r) for r in range(2)], 1)
+    M = A.to_scipy_full()
+    for prec, tol in (("fp64", 1e-13), ("fp32", 1e-10), ("fp32", 1e-13)):
+        rc, Y, _ = T.linsolve(A, B, ["taucs.factor.LLT=true", "taucs.factor.mf=true",
+                                     f"taucs.factor.precision={prec}", "taucs.solve.cg=true",
+                                     f"taucs.solve.convergetol={tol}"])
+        rr = np.linalg.norm(B - M @ Y, axis=0) / np.linalg.norm(B, axis=0)
+        assert rc == T.TAUCS_SUCCESS and rr.max() <= tol, (prec, tol, rr)
+
+
+def test_linsolve_cg_with_a_kept_factor_on_a_new_matrix():
+    """The reference's flow of reusing a factor as a preconditioner:
+    taucs.factor=false + taucs.solve.cg=true on a matrix whose values differ
+    from the factored one.  The residuals must be those of the new matrix,
+    even when the new permuted matrix lands at the address of the old one."""
+    print("case: linsolve cg, kept factor, new values", flush=True)
+    A = T.mesh3d_varcoef(14, 14, 14, seed=3)
+    n, cp, ri, v = A.arrays()
+    rc, _, F = T.linsolve(A, None, ["taucs.factor.LLT=true", "taucs.factor.mf=true"])
+    assert rc == T.TAUCS_SUCCESS and F
+    B = np.stack([T.random_uniform(n, 60 + r) for r in range(2)], 1)
+    d = 1.0 + 0.2 * np.random.default_rng(0).random(n)
+    col = np.repeat(np.arange(n), np.diff(cp))
+    for scale in ("uniform", "diagonal"):
+        # 2A, and D A D with a random positive diagonal D: both stay SPD
+        v2 = 2.0 * v if scale == "uniform" else v * d[ri] * d[col]
+        A2 = T.from_arrays(n, cp, ri, v2)
+        M2 = A2.to_scipy_full()
+        rc, Y, _ = T.linsolve(A2, B, ["taucs.factor=false", "taucs.solve.cg=true",
+                                      "taucs.solve.convergetol=1e-12", "taucs.solve.maxits=200"], F=F)
+        rr = np.linalg.norm(B - M2 @ Y, axis=0) / np.linalg.norm(B, axis=0)
+        assert rc == T.TAUCS_SUCCESS and rr.max() <= 1e-12, (scale, rr)
+    T.linsolve(None, None, [], F=F)
+
+
+def test_refinement_reads_values_changed_in_place():
+    """solve_refined without the factored-values flag uploads A's values on
+    every call: changing them in place (same pointer) changes the residuals."""
+    print("case: refine after in-place value change", flush=True)
+    A = T.mesh3d_varcoef(12, 12, 12, seed=8)
+    _, _, PA = prepared(A)
+    n = PA.n
+    F = T.factor_llt_mf(PA)
+    b = T.random_uniform(n, 9)
+    x1, i1 = F.solve_refined(PA, b, method=0, tol=1e-14, maxit=20)
+    assert i1.converged
+    vals = np.ctypeslib.as_array(T.C.cast(PA.ptr.contents.values, T.C.POINTER(T.C.c_double)),
+                                 shape=(PA.nnz,))
+    vals *= 1.5          # in place: same matrix object, same value pointer
+    x2, i2 = F.solve_refined(PA, b, method=1, tol=1e-14, maxit=100)
+    assert i2.converged
+    M = PA.to_scipy_full()
+    assert np.linalg.norm(b - M @ x2) / np.linalg.norm(b) <= 1e-12
+    assert np.allclose(x2, x1 / 1.5, rtol=1e-9, atol=1e-12)
+
+
 @pytest.mark.parametrize("prec", [T.FP32])
 @pytest.mark.parametrize("method", [0, 1])
 def test_mixed_precision_refinement(prec, method):

@@ -42,7 +42,7 @@ class FakeFactor:
         time.sleep(self.delay)
         return 0
 
-    def solve_refined(self, PA, B, method=0, tol=1e-12, maxit=50):
+    def solve_refined(self, PA, B, method=0, tol=1e-12, maxit=50, factored=False):
         assert not self.active, "two threads on one factor"
         assert not self.freed, "solve on an evicted factor"
         self.active = True
