@@ -39,6 +39,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 /* ---- generators ----------------------------------------------------------- */
 
@@ -612,3 +613,304 @@ void oracle_sym_free(oracle_sym* L)
 }
 
 void oracle_free(void* p) { free(p); }
+
+/* ---- numeric phase: the multifrontal factor and the supernodal solve ------
+ *
+ * A C restatement of the reference's numeric path, for the CPU baseline and
+ * fast oracle solutions of large systems:
+ *   oracle_mf_factor   recursive_multifrontal_supernodal_factor_llt
+ *                      (src/taucs_sn_llt.c:1831-1977): children in child-list
+ *                      order, each extend-added into the parent front as it
+ *                      returns (extend_add_wrapper :1797-1827, the front
+ *                      created zeroed at the first child, :1013-1047); then
+ *                      multifrontal_supernodal_front_factor (:1086-1194):
+ *                      A's columns added through the row bitmap, dpotrf on F1
+ *                      (s x s), dtrsm R/L/T/N on F2 (u x s), dsyrk L/N into U
+ *   oracle_mf_solve    taucs_dsupernodal_solve_llt (:2844-2885): forward
+ *                      recursion :2597-2720 (gather, trsm L/L/N/N, gemm N/N,
+ *                      scatter), backward :2723-2841 (gather, gemm T/N,
+ *                      trsm L/L/T/N, scatter), here with k right-hand sides as
+ *                      the columns of the dense blocks
+ * The dense kernels are the host's BLAS/LAPACK (the reference links the
+ * Fortran BLAS), resolved at run time from a shared library given by path
+ * (scipy's bundled OpenBLAS exports them as scipy_dpotrf_ etc.). */
+
+typedef void (*potrf_fn)(const char*, const int*, double*, const int*, int*);
+typedef void (*trsm_fn)(const char*, const char*, const char*, const char*, const int*, const int*,
+                        const double*, const double*, const int*, double*, const int*);
+typedef void (*syrk_fn)(const char*, const char*, const int*, const int*, const double*,
+                        const double*, const int*, const double*, double*, const int*);
+typedef void (*gemm_fn)(const char*, const char*, const int*, const int*, const int*, const double*,
+                        const double*, const int*, const double*, const int*, const double*, double*,
+                        const int*);
+static potrf_fn b_potrf;
+static trsm_fn b_trsm;
+static syrk_fn b_syrk;
+static gemm_fn b_gemm;
+static void (*b_threads)(int);
+static const char* (*b_config)(void);
+
+int oracle_load_blas(const char* so_path, const char* prefix)
+{
+  char nm[128];
+  void* h = dlopen(so_path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return -1;
+#define SYM(var, type, base)                                  \
+  snprintf(nm, sizeof nm, "%s%s", prefix, base);              \
+  var = (type)dlsym(h, nm);                                   \
+  if (!var) return -1;
+  SYM(b_potrf, potrf_fn, "dpotrf_")
+  SYM(b_trsm, trsm_fn, "dtrsm_")
+  SYM(b_syrk, syrk_fn, "dsyrk_")
+  SYM(b_gemm, gemm_fn, "dgemm_")
+#undef SYM
+  snprintf(nm, sizeof nm, "%sopenblas_set_num_threads", prefix);
+  b_threads = (void (*)(int))dlsym(h, nm);
+  snprintf(nm, sizeof nm, "%sopenblas_get_config", prefix);
+  b_config = (const char* (*)(void))dlsym(h, nm);
+  return 0;
+}
+
+/* the BLAS library's own description of itself (OpenBLAS: version and build) */
+const char* oracle_blas_config(void) { return b_config ? b_config() : "unknown"; }
+
+int oracle_blas_threads(int t)
+{
+  if (!b_threads) return -1;
+  b_threads(t);
+  return 0;
+}
+
+typedef struct {
+  const oracle_sym* S;
+  int n;
+  double** F1;   /* [n_sn] s x s, ld s (lower: L11) */
+  double** F2;   /* [n_sn] u x s, ld u (L21) */
+  int failed_sn; /* -1, or the supernode whose pivot was not positive */
+} oracle_num;
+
+typedef struct { double *F1, *F2, *U; } front_t;
+
+/* wall time of the last factorization by part (s): assembly, dpotrf, dtrsm,
+   dsyrk, extend-add, front allocation */
+static double t_part[6];
+static double now_s(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+void oracle_mf_times(double* out) { memcpy(out, t_part, sizeof t_part); }
+
+static int front_create(const oracle_sym* S, int sn, front_t* f)
+{
+  size_t s = (size_t)S->sn_size[sn], u = (size_t)(S->sn_up_size[sn] - S->sn_size[sn]);
+  f->F1 = calloc(s * s + 1, sizeof(double));
+  f->F2 = calloc(u * s + 1, sizeof(double));
+  f->U = calloc(u * u + 1, sizeof(double));
+  return (f->F1 && f->F2 && f->U) ? 0 : -1;
+}
+
+/* child's U (lower) into the parent front, entries swapped to stay lower */
+static void extend_add(const oracle_sym* S, int parent, front_t* pf, int child, const double* U,
+                       int* bitmap)
+{
+  const int ps = S->sn_size[parent], pu = S->sn_up_size[parent] - ps;
+  const int cs = S->sn_size[child], cu = S->sn_up_size[child] - cs;
+  const int* prow = S->sn_struct[parent];
+  const int* crow = S->sn_struct[child] + cs;
+  int i, j;
+  for (i = 0; i < ps + pu; i++) bitmap[prow[i]] = i;
+  for (j = 0; j < cu; j++)
+    for (i = j; i < cu; i++) {
+      int pj = bitmap[crow[j]], pi = bitmap[crow[i]];
+      double v = U[(size_t)cu * j + i];
+      if (pj > pi) { int t = pj; pj = pi; pi = t; }
+      if (pj < ps) {
+        if (pi < ps) pf->F1[(size_t)ps * pj + pi] += v;
+        else pf->F2[(size_t)pu * pj + (pi - ps)] += v;
+      } else {
+        pf->U[(size_t)pu * (pj - ps) + (pi - ps)] += v;
+      }
+    }
+}
+
+static int front_factor(const oracle_sym* S, int sn, front_t* f, const int* colptr,
+                        const int* rowind, const double* vals, int* bitmap)
+{
+  const int s = S->sn_size[sn], u = S->sn_up_size[sn] - s;
+  const int* rows = S->sn_struct[sn];
+  int i, j, info = 0;
+  const double one = 1.0, mone = -1.0;
+  double t0 = now_s(), t1, t2, t3;
+  for (i = 0; i < s + u; i++) bitmap[rows[i]] = i;
+  for (j = 0; j < s; j++) {
+    const int c = rows[j];
+    for (i = colptr[c]; i < colptr[c + 1]; i++) {
+      const int r = bitmap[rowind[i]];
+      if (r < s) f->F1[(size_t)s * j + r] += vals[i];
+      else f->F2[(size_t)u * j + (r - s)] += vals[i];
+    }
+  }
+  t1 = now_s();
+  if (s) b_potrf("L", &s, f->F1, &s, &info);
+  t2 = now_s();
+  t_part[0] += t1 - t0;
+  t_part[1] += t2 - t1;
+  if (info) return -1;
+  if (u && s) {
+    b_trsm("R", "L", "T", "N", &u, &s, &one, f->F1, &s, f->F2, &u);
+    t3 = now_s();
+    b_syrk("L", "N", &u, &s, &mone, f->F2, &u, &one, f->U, &u);
+    t_part[2] += t3 - t2;
+    t_part[3] += now_s() - t3;
+  }
+  return 0;
+}
+
+/* live supernodes, children before parents, children in child-list order */
+static int* postorder_sn(const oracle_sym* S, int* count)
+{
+  int* out = malloc(sizeof(int) * (S->n_sn + 1));
+  int* st = malloc(sizeof(int) * (S->n_sn + 2));
+  int* it = malloc(sizeof(int) * (S->n_sn + 2));
+  int top = 0, k = 0;
+  st[0] = S->n_sn;
+  it[0] = S->first_child[S->n_sn];
+  while (top >= 0) {
+    int c = it[top];
+    if (c == -1) {
+      if (st[top] != S->n_sn) out[k++] = st[top];
+      top--;
+    } else {
+      it[top] = S->next_child[c];
+      st[++top] = c;
+      it[top] = S->first_child[c];
+    }
+  }
+  free(st);
+  free(it);
+  *count = k;
+  return out;
+}
+
+oracle_num* oracle_mf_factor(const oracle_sym* S, const int* colptr, const int* rowind,
+                             const double* vals)
+{
+  int cnt, k, *order, *bitmap, *parent;
+  front_t* fr;
+  oracle_num* N;
+  if (!b_potrf || !S || S->failed) return NULL;
+  N = calloc(1, sizeof(oracle_num));
+  N->S = S;
+  N->n = S->n;
+  N->failed_sn = -1;
+  N->F1 = calloc(S->n_sn + 1, sizeof(double*));
+  N->F2 = calloc(S->n_sn + 1, sizeof(double*));
+  fr = calloc(S->n_sn + 1, sizeof(front_t));
+  bitmap = malloc(sizeof(int) * (S->n + 1));
+  parent = malloc(sizeof(int) * (S->n_sn + 1));
+  for (k = 0; k <= S->n_sn; k++) parent[k] = -1;
+  for (k = 0; k <= S->n_sn; k++) {
+    int c;
+    for (c = S->first_child[k]; c != -1; c = S->next_child[c]) parent[c] = k;
+  }
+  order = postorder_sn(S, &cnt);
+  memset(t_part, 0, sizeof t_part);
+  for (k = 0; k < cnt; k++) {
+    const int sn = order[k], p = parent[sn];
+    double ta = now_s();
+    if (!fr[sn].F1 && front_create(S, sn, &fr[sn]) != 0) { N->failed_sn = sn; break; }
+    t_part[5] += now_s() - ta;
+    if (front_factor(S, sn, &fr[sn], colptr, rowind, vals, bitmap) != 0) {
+      N->failed_sn = sn;
+      break;
+    }
+    N->F1[sn] = fr[sn].F1;
+    N->F2[sn] = fr[sn].F2;
+    if (p != S->n_sn && p >= 0) {   /* the virtual root takes no extend-add */
+      double tb = now_s(), tc;
+      if (!fr[p].F1 && front_create(S, p, &fr[p]) != 0) { N->failed_sn = p; break; }
+      tc = now_s();
+      extend_add(S, p, &fr[p], sn, fr[sn].U, bitmap);
+      t_part[5] += tc - tb;
+      t_part[4] += now_s() - tc;
+    }
+    free(fr[sn].U);
+    fr[sn].U = NULL;
+  }
+  for (k = 0; k <= S->n_sn; k++) {
+    free(fr[k].U);
+    if (fr[k].F1 && N->F1[k] != fr[k].F1) { free(fr[k].F1); free(fr[k].F2); }
+  }
+  free(fr); free(bitmap); free(parent); free(order);
+  return N;
+}
+
+int oracle_mf_failed(const oracle_num* N) { return N ? N->failed_sn : -2; }
+
+/* X (n x k, ld n) = (L L^T)^{-1} B in the permuted space */
+int oracle_mf_solve(const oracle_num* N, int k, const double* B, double* X)
+{
+  const oracle_sym* S = N->S;
+  const int n = N->n;
+  int cnt, q, i, r, *order = postorder_sn(S, &cnt);
+  double* b = malloc(sizeof(double) * (size_t)n * k);   /* forward: x <- b, updated */
+  double* y = malloc(sizeof(double) * (size_t)n * k);   /* forward solution */
+  double* t = malloc(sizeof(double) * (size_t)(n + 1) * k);
+  const double one = 1.0, mone = -1.0, zero = 0.0;
+  if (N->failed_sn != -1) { free(order); free(b); free(y); free(t); return -1; }
+  memcpy(b, B, sizeof(double) * (size_t)n * k);
+  for (q = 0; q < cnt; q++) {
+    const int sn = order[q], s = S->sn_size[sn], u = S->sn_up_size[sn] - s;
+    const int* rows = S->sn_struct[sn];
+    double *xd = t, *bd = t + (size_t)s * k;
+    for (r = 0; r < k; r++)
+      for (i = 0; i < s; i++) xd[i + (size_t)s * r] = b[rows[i] + (size_t)n * r];
+    b_trsm("L", "L", "N", "N", &s, &k, &one, N->F1[sn], &s, xd, &s);
+    if (u > 0 && s > 0)
+      b_gemm("N", "N", &u, &k, &s, &one, N->F2[sn], &u, xd, &s, &zero, bd, &u);
+    for (r = 0; r < k; r++) {
+      for (i = 0; i < s; i++) y[rows[i] + (size_t)n * r] = xd[i + (size_t)s * r];
+      for (i = 0; i < u; i++) b[rows[s + i] + (size_t)n * r] -= bd[i + (size_t)u * r];
+    }
+  }
+  for (q = cnt - 1; q >= 0; q--) {
+    const int sn = order[q], s = S->sn_size[sn], u = S->sn_up_size[sn] - s;
+    const int* rows = S->sn_struct[sn];
+    double *bd = t, *xd = t + (size_t)s * k;
+    for (r = 0; r < k; r++) {
+      for (i = 0; i < s; i++) bd[i + (size_t)s * r] = y[rows[i] + (size_t)n * r];
+      for (i = 0; i < u; i++) xd[i + (size_t)u * r] = X[rows[s + i] + (size_t)n * r];
+    }
+    if (u > 0 && s > 0)
+      b_gemm("T", "N", &s, &k, &u, &mone, N->F2[sn], &u, xd, &u, &one, bd, &s);
+    b_trsm("L", "L", "T", "N", &s, &k, &one, N->F1[sn], &s, bd, &s);
+    for (r = 0; r < k; r++)
+      for (i = 0; i < s; i++) X[rows[i] + (size_t)n * r] = bd[i + (size_t)s * r];
+  }
+  free(order); free(b); free(y); free(t);
+  return 0;
+}
+
+/* panel of supernode sn as [L11; L21], (s+u) x s column-major, ld s+u */
+int oracle_mf_panel(const oracle_num* N, int sn, double* out)
+{
+  const oracle_sym* S = N->S;
+  const int s = S->sn_size[sn], u = S->sn_up_size[sn] - s, m = s + u;
+  int i, j;
+  if (!N->F1[sn]) return -1;
+  for (j = 0; j < s; j++) {
+    for (i = 0; i < s; i++) out[i + (size_t)m * j] = i >= j ? N->F1[sn][i + (size_t)s * j] : 0.0;
+    for (i = 0; i < u; i++) out[s + i + (size_t)m * j] = N->F2[sn][i + (size_t)u * j];
+  }
+  return 0;
+}
+
+void oracle_mf_free(oracle_num* N)
+{
+  int k;
+  if (!N) return;
+  for (k = 0; k <= N->S->n_sn; k++) { free(N->F1[k]); free(N->F2[k]); }
+  free(N->F1); free(N->F2); free(N);
+}

@@ -50,6 +50,16 @@ def lib():
             getattr(_lib, g).argtypes = [C.c_int] + [C.c_void_p] * 3
         _lib.oracle_mesh3d.argtypes = [C.c_int] * 3 + [C.c_void_p] * 3
         _lib.oracle_free.argtypes = [C.c_void_p]
+        _lib.oracle_load_blas.argtypes = [C.c_char_p, C.c_char_p]
+        _lib.oracle_blas_threads.argtypes = [C.c_int]
+        _lib.oracle_mf_factor.restype = C.c_void_p
+        _lib.oracle_mf_factor.argtypes = [C.c_void_p] * 4
+        _lib.oracle_mf_failed.argtypes = [C.c_void_p]
+        _lib.oracle_mf_solve.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        _lib.oracle_mf_panel.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        _lib.oracle_mf_free.argtypes = [C.c_void_p]
+        _lib.oracle_mf_times.argtypes = [C.c_void_p]
+        _lib.oracle_blas_config.restype = C.c_char_p
     return _lib
 
 
@@ -367,6 +377,95 @@ def ccs_solve_llt(n, colptr, rowind, vals, b):
         y[i] -= np.dot(x[rowind[p0 + 1:p1]], vals[p0 + 1:p1])
         x[i] = y[i] / vals[p0]
     return x
+
+
+_blas_loaded = False
+
+
+def _load_blas():
+    """The host BLAS/LAPACK of the C numeric oracle: scipy's bundled OpenBLAS
+    (exported as scipy_dpotrf_, ...)."""
+    global _blas_loaded
+    if _blas_loaded:
+        return
+    import glob
+    import scipy
+    libs = sorted(glob.glob(os.path.join(os.path.dirname(scipy.__file__), "..", "scipy.libs",
+                                         "libscipy_openblas*.so")))
+    if not libs or lib().oracle_load_blas(os.path.abspath(libs[0]).encode(), b"scipy_") != 0:
+        raise RuntimeError("no host BLAS/LAPACK for the C numeric oracle")
+    _blas_loaded = True
+
+
+def blas_config():
+    _load_blas()
+    return lib().oracle_blas_config().decode(errors="replace")
+
+
+def blas_threads(t):
+    """Threads of the host BLAS used by the C numeric oracle."""
+    _load_blas()
+    lib().oracle_blas_threads(int(t))
+
+
+class MF:
+    """The C restatement of the numeric path (oracle.c: oracle_mf_factor /
+    oracle_mf_solve): the reference's multifrontal factor with dpotrf / dtrsm /
+    dsyrk per front and its supernodal solve, on the host BLAS.  Construction
+    runs the symbolic phase and the numeric factorization; `t_symbolic` and
+    `t_numeric` are their wall times (s)."""
+
+    def __init__(self, n, colptr, rowind, vals):
+        import time
+        _load_blas()
+        L = lib()
+        self.colptr = np.ascontiguousarray(colptr, np.int32)
+        self.rowind = np.ascontiguousarray(rowind, np.int32)
+        self.vals = np.ascontiguousarray(vals, np.float64)
+        self.n = n
+        t0 = time.perf_counter()
+        self.S = L.oracle_symbolic(n, _p(self.colptr), _p(self.rowind), 0)
+        t1 = time.perf_counter()
+        if not self.S:
+            raise RuntimeError("oracle symbolic failed")
+        self.N = L.oracle_mf_factor(self.S, _p(self.colptr), _p(self.rowind), _p(self.vals))
+        t2 = time.perf_counter()
+        self.t_symbolic, self.t_numeric = t1 - t0, t2 - t1
+        if not self.N or L.oracle_mf_failed(self.N) != -1:
+            self.free()
+            raise np.linalg.LinAlgError("not positive definite")
+        ns, dp, fl = C.c_int(), C.c_int(), C.c_int()
+        st = np.zeros(4)
+        L.oracle_sym_info(self.S, C.byref(ns), C.byref(dp), C.byref(fl), _p(st))
+        self.fund_flops, self.relax_flops = st[1], st[3]
+        tp = np.zeros(6)
+        L.oracle_mf_times(_p(tp))
+        self.parts = dict(zip(("assemble", "potrf", "trsm", "syrk", "extend_add", "alloc"), tp))
+
+    def solve(self, b):
+        b = np.asarray(b, dtype=np.float64)
+        B = np.asfortranarray(b.reshape(b.shape[0], -1))
+        X = np.zeros_like(B, order="F")
+        if lib().oracle_mf_solve(self.N, B.shape[1], _p(B), _p(X)) != 0:
+            raise RuntimeError("oracle solve failed")
+        return X.reshape(b.shape)
+
+    def panel(self, sn, m, s):
+        out = np.zeros((m, s), order="F")
+        if lib().oracle_mf_panel(self.N, sn, _p(out)) != 0:
+            raise KeyError(sn)
+        return out
+
+    def free(self):
+        if getattr(self, "N", None):
+            lib().oracle_mf_free(self.N)
+            self.N = None
+        if getattr(self, "S", None):
+            lib().oracle_sym_free(self.S)
+            self.S = None
+
+    def __del__(self):
+        self.free()
 
 
 def residual(n, colptr, rowind, vals, x, b):

@@ -266,3 +266,44 @@ def test_ccs_solve_llt_matches_dense_triangular_solves(seed):
     x = O.ccs_solve_llt(n, np.array(cp), np.array(ri), np.array(v), b)
     xd = sl.solve_triangular(L, sl.solve_triangular(L, b, lower=True), lower=True, trans="T")
     assert np.abs(x - xd).max() <= 1e-12 * np.abs(xd).max()
+
+
+def test_c_numeric_oracle_matches_dense_cholesky_and_the_python_restatement():
+    """oracle.c's multifrontal factor (host BLAS per front, F1/F2 kept apart as
+    the reference keeps them) and its supernodal solve: every panel equals the
+    dense Cholesky factor of P A P^T, and the solution equals a dense solve
+    and the numpy restatement's."""
+    for n, cp, ri, v in (O.mesh2d(9), O.mesh3d(5, 5, 5), random_spd_pattern(45, 0.1, 5),
+                         O.mesh3d_varcoef(4, 5, 6, 9)):
+        inv = O.genmmd(n, cp, ri)[1]
+        pcp, pri, pv = O.permute(n, cp, ri, v, inv)
+        sym = O.symbolic(n, pcp, pri)
+        M = O.MF(n, pcp, pri, pv)
+        Ad = dense(n, pcp, pri, pv)
+        Lref = np.linalg.cholesky(Ad)
+        for sn in O.postorder(sym):
+            rows = O.rows_of(sym, sn)
+            s = int(sym["sn_size"][sn])
+            P = M.panel(sn, len(rows), s)
+            for jj in range(s):
+                # |l_ij| <= sqrt(a_ii): rounding of the two factorizations differs
+                # by a few ulps of that scale (a tiny last pivot cancels)
+                ref = Lref[rows[jj:], rows[jj]]
+                assert np.abs(P[jj:, jj] - ref).max() <= 1e-12 * np.sqrt(np.diag(Ad).max())
+        b = np.random.default_rng(3).uniform(-1, 1, (n, 4))
+        x = M.solve(b)
+        # agreement to about cond(A) * u (the variable-coefficient case has
+        # cond ~ 1e7: solutions of size 3e4 differ in the 1e-10 digit)
+        xd = np.linalg.solve(Ad, b)
+        xp = O.solve(sym, O.factor(sym, pcp, pri, pv), b)
+        tol = 1e-16 * np.linalg.cond(Ad) * np.abs(xd).max() + 1e-13
+        assert np.abs(x - xd).max() <= tol and np.abs(x - xp).max() <= tol
+        M.free()
+
+
+def test_c_numeric_oracle_rejects_indefinite():
+    n, cp, ri, v = O.mesh2d(6)
+    v = v.copy()
+    v[cp[10 + 1] - 1] = -5.0
+    with pytest.raises(np.linalg.LinAlgError):
+        O.MF(n, cp, ri, v)
