@@ -95,6 +95,24 @@ def host_backward_error(M, X, B):
     return float((abs(R).max(0) / (anorm * abs(X).max(0) + abs(B).max(0))).max())
 
 
+def oracle_forward_error(T, X, gold):
+    """Relative forward error of X against the oracle's FP64 factor-and-solve,
+    from the committed fixture (the oracle's x through a 32-row random-sign
+    sketch, and at 4096 sampled rows): |S(x - x_oracle)| / |S x_oracle| per
+    column, which estimates |x - x_oracle| / |x_oracle| within a factor of
+    about 1.5, and max_i |x_i - x_oracle,i| / |x_oracle| over the sampled rows.
+    S is drawn from the same splitmix64 stream as the fixture's (seed
+    777 + 7919 i), here with the product's generator."""
+    X = np.asarray(X).reshape(X.shape[0], -1)
+    n = X.shape[0]
+    SX = np.empty((32, X.shape[1]))
+    for i in range(32):
+        SX[i] = np.where(T.random_uniform(n, 777 + 7919 * i, 0.0, 1.0) < 0.5, 1.0, -1.0) @ X
+    sk = np.linalg.norm(SX - gold["sketch"], axis=0) / np.linalg.norm(gold["sketch"], axis=0)
+    sa = np.abs(X[gold["sample_idx"]] - gold["sample_x"]).max(0) / gold["xnorm"]
+    return {"sketch": float(sk.max()), "sampled_rows": float(sa.max())}
+
+
 def bench_varcoef(T, side=80, nrhs=4, steps=2, seed=2024):
     """Variable-coefficient 3D operator (log-uniform conductivities over
     [1e-2, 1e2], shift 1e-6): plain FP64 factor + solve against FP32 fronts +
@@ -156,10 +174,14 @@ def bench_varcoef(T, side=80, nrhs=4, steps=2, seed=2024):
         (w, t), _ = sched[prec]
         F.autotune(PA, prec, candidates=((w, t),), reps=1)
 
+    gp = os.path.join(ROOT, "tests", "golden", "c3_varcoef_80.npz")
+    gold = np.load(gp) if side == 80 and seed == 2024 and nrhs == 4 and os.path.exists(gp) else None
     use(T.FP64)
     r64 = run(T.FP64, None, 0, 0)
     out["fp64"] = r64
     x64 = r64["X"]
+    if gold is not None:
+        r64["forward_error_vs_oracle_fp64"] = oracle_forward_error(T, x64, gold)
     base = r64["factor_ms"] + r64["solve_ms"]
     for name, prec, method in (("fp32_ir", T.FP32, 0), ("fp32_pcg", T.FP32, 1),
                                ("bf16upd_ir", T.BF16UPD, 0), ("bf16upd_pcg", T.BF16UPD, 1)):
@@ -170,6 +192,9 @@ def bench_varcoef(T, side=80, nrhs=4, steps=2, seed=2024):
             log(f"varcoef {name}: breakdown")
             continue
         r["forward_error_vs_fp64"] = float(np.linalg.norm(r["X"] - x64) / np.linalg.norm(x64))
+        if gold is not None:
+            r["forward_error_vs_oracle_fp64"] = oracle_forward_error(T, r["X"], gold)
+        r["fp64_fallback_blocks"] = T.lib.taucs_amd_factor_fp64_fallbacks(F.h)
         r["speedup_factor_plus_solve_vs_fp64"] = round(base / (r["factor_ms"] + r["solve_ms"]), 3)
         out[name] = r
         log(f"varcoef {name}: {({k: v for k, v in r.items() if k != 'X'})}")
@@ -260,20 +285,98 @@ def bench_batch(T, count=512, lo=16, hi=128, steps=3, seed=7):
     return out
 
 
-def bench_serving(T, requests=300, rate=40.0, slots=4, budget_gib=16.0, seed=3):
-    """Poisson request mix through the serving loop, FP64 and FP32 + refinement."""
+def bench_serving(T, requests=300, slots=4, budget_gib=128.0, seed=3, loads=(0.5, 0.8, 0.95)):
+    """Poisson request mix through the serving loop, FP64 and FP32 + refinement.
+    Capacity first: the same requests all arriving at once (a saturated queue),
+    requests/s over the drain.  Then the mix at 50/80/95% of that capacity,
+    with p50/p99 latency and solves/s at each offered load."""
     from taucs_amd import serving as S
     out = {}
     for name, prec in (("fp64", T.FP64), ("fp32_ir", T.FP32)):
-        mix = S.make_mix(requests, rate, seed=seed)
-        srv = S.Server(hbm_budget=int(budget_gib * S.GIB), slots=slots, precision=prec, tol=1e-12)
-        summ = S.run_mix(mix, srv)
-        srv.close()
-        out[name] = {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in summ.items()}
-        log(f"serving {name}: {out[name]}")
-    out["workload"] = (f"{requests} requests, Poisson {rate}/s: 70% 2D side U[32,256], 30% 3D side "
-                       f"U[8,40], 1-16 rhs, 40% solve-only; {slots} slots, budget {budget_gib} GiB")
+        def one(rate):
+            mix = S.make_mix(requests, rate, seed=seed)
+            srv = S.Server(hbm_budget=int(budget_gib * S.GIB), slots=slots, precision=prec, tol=1e-12)
+            summ = S.run_mix(mix, srv)
+            srv.close()
+            return {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in summ.items()}
+
+        sat = one(1e9)
+        cap = sat["requests_per_s"]
+        res = {"capacity_requests_per_s": cap, "capacity_solves_per_s": sat["solves_per_s"],
+               "saturated": sat, "loads": {}}
+        log(f"serving {name}: capacity {cap:.1f} requests/s ({sat['solves_per_s']:.0f} solves/s)")
+        for f in loads:
+            r = one(f * cap)
+            r["offered_requests_per_s"] = round(f * cap, 2)
+            res["loads"][f"{int(round(100 * f))}%"] = r
+            log(f"serving {name} at {100 * f:.0f}% of capacity: {r}")
+        out[name] = res
+    out["workload"] = (f"{requests} requests: 70% 2D side U[32,256], 30% 3D side U[8,40], 1-16 rhs, "
+                       f"40% solve-only; Poisson arrivals at 50/80/95% of the measured capacity; "
+                       f"{slots} slots, HBM budget {budget_gib} GiB")
     return out
+
+
+def profile_workload(T, name, steps, panel_width, gemm_tile):
+    """Only one workload's factor + solve steps, at a fixed schedule, for a
+    profiler run (kernel trace or counters): varcoef-{fp64,fp32,bf16upd}
+    (mesh3d_varcoef 80^3, seed 2024, 4 rhs; mixed precision refined to
+    backward error 1e-12 by CG) or batch-{fp64,fp32} (the 512 shifted 2D
+    systems, side U[16,128]; FP32 refined to 1e-13)."""
+    kind, prec_name = name.split("-")
+    prec = {"fp64": T.FP64, "fp32": T.FP32, "bf16upd": T.BF16UPD}[prec_name]
+    if panel_width:
+        T.lib.taucs_amd_set_panel_width(panel_width)
+        T.lib.taucs_amd_set_gemm_tile(gemm_tile or 64)
+    if kind == "varcoef":
+        A = T.mesh3d_varcoef(80, 80, 80, 2024)
+        perm, inv = T.order(A, "genmmd")
+        PA = T.permute(A, perm, inv)
+        F = T.factor_symbolic(PA)
+        B = np.stack([T.random_uniform(PA.n, 100 + r) for r in range(4)], 1)
+        dV = T.DeviceArray(PA.arrays()[3])
+        dB, dX = T.DeviceArray(B), T.DeviceArray(np.zeros_like(B))
+        for i in range(steps):
+            t0 = time.perf_counter()
+            if F.numeric_device(dV.ptr, prec) != 0:
+                raise RuntimeError("factorization failed")
+            t1 = time.perf_counter()
+            if prec == T.FP64:
+                F.solve_device(dX.ptr, dB.ptr, PA.n, 4)
+                its = 0
+            else:
+                _, info = F.solve_refined(PA, B, method=1, tol=1e-12, maxit=100, factored=True)
+                its = info.iterations
+            t2 = time.perf_counter()
+            log(f"{name} step {i}: factor {1e3 * (t1 - t0):.2f} ms, solve {1e3 * (t2 - t1):.2f} ms, "
+                f"iterations {its}")
+        F.free()
+    elif kind == "batch":
+        rng = np.random.default_rng(7)
+        sides = rng.integers(16, 129, 512)
+        mats = []
+        for q, s in enumerate(sides):
+            A = T.mesh2d_shifted(int(s), seed=10_000 + q)
+            perm, inv = T.order(A, "genmmd")
+            mats.append(T.permute(A, perm, inv))
+        Bt = T.Batch(mats, precision=prec)
+        b = np.concatenate([T.random_uniform(m.n, 500 + q) for q, m in enumerate(mats)])
+        dV = T.DeviceArray(np.concatenate([m.arrays()[3] for m in mats]))
+        dB, dX = T.DeviceArray(b.reshape(-1, 1)), T.DeviceArray(np.zeros((b.size, 1)))
+        for i in range(steps):
+            t0 = time.perf_counter()
+            if Bt.numeric_device(dV.ptr, prec) != 0:
+                raise RuntimeError("batched factorization failed")
+            t1 = time.perf_counter()
+            if prec == T.FP64:
+                Bt.solve_device(dX.ptr, dB.ptr, 1)
+            else:
+                Bt.solve_refined(b, method=0, tol=1e-13, maxit=50)
+            t2 = time.perf_counter()
+            log(f"{name} step {i}: factor {1e3 * (t1 - t0):.2f} ms, solve {1e3 * (t2 - t1):.2f} ms")
+        Bt.free()
+    else:
+        raise SystemExit(f"unknown workload {name}")
 
 
 def main():
@@ -297,6 +400,10 @@ def main():
                          "the reference point for hardware byte counters under a profiler")
     ap.add_argument("--dump-ops", default=None, help="write the per-op profile of the FP64 "
                     "factorization (kind, level, tasks, flops, bytes, ms) to this JSON file")
+    ap.add_argument("--profile-workload", default=None,
+                    help="run only this workload's steps for a profiler: varcoef-fp64, varcoef-fp32, "
+                         "varcoef-bf16upd, batch-fp64 or batch-fp32 (with --steps, --panel-width, "
+                         "--gemm-tile); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
@@ -304,6 +411,9 @@ def main():
         sys.exit(2)
 
     import taucs_amd as T
+    if args.profile_workload:
+        profile_workload(T, args.profile_workload, args.steps, args.panel_width, args.gemm_tile)
+        return
     side, k = args.side, args.nrhs
     log(f"bench: mesh3d {side}^3, {k} rhs, steps {args.steps} warmup {args.warmup}")
     A = T.mesh3d(side)

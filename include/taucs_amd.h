@@ -85,6 +85,15 @@ int   taucs_amd_factor_llt_numeric_precision(taucs_ccs_matrix* A, void* L, int p
 /* Numeric factorization from A's values already in device memory (nnz
    doubles in the stored order of the permuted matrix of the symbolic factor). */
 int   taucs_amd_factor_llt_numeric_device(void* L, const double* d_vals, int precision);
+/* FP64 fallback of reduced-precision factors (default on): a 64 x 64 diagonal
+   block (or a whole front of at most 64 rows) whose FP32 Cholesky meets a
+   nonpositive pivot, or a pivot below 2^-12 of the block's input diagonal
+   entry, is refactored in FP64 from its input and rounded back to FP32; the
+   factorization fails only if the FP64 Cholesky fails too.  Applies to
+   schedules launched afterwards.  The count is that of the factor's last
+   numeric factorization (batch handles included), -1 without one. */
+void  taucs_amd_set_fp64_fallback(int on);
+int   taucs_amd_factor_fp64_fallbacks(void* L);
 
 typedef struct {
   int    iterations;     /* outer iterations used (max over right-hand sides) */

@@ -100,6 +100,8 @@ _sigs = {
                                                   C.c_void_p, C.c_int]),
     "taucs_amd_supernodal_solve_many_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                                          C.c_void_p, C.c_int]),
+    "taucs_amd_set_fp64_fallback": (None, [C.c_int]),
+    "taucs_amd_factor_fp64_fallbacks": (C.c_int, [C.c_void_p]),
     "taucs_amd_factor_llt_mf_precision": (C.c_void_p, [_ccs_p, C.c_int]),
     "taucs_amd_factor_llt_numeric_precision": (C.c_int, [_ccs_p, C.c_void_p, C.c_int]),
     "taucs_amd_solve_refined": (C.c_int, [C.c_void_p, _ccs_p, C.c_int, C.c_void_p, C.c_int,
@@ -232,9 +234,14 @@ def from_arrays(n, colptr, rowind, vals, flags=TAUCS_SYMMETRIC | TAUCS_LOWER | T
     if not p:
         raise MemoryError
     A = p.contents
-    C.memmove(A.colptr, np.ascontiguousarray(colptr, dtype=np.int32).ctypes.data, 4 * (n + 1))
-    C.memmove(A.rowind, np.ascontiguousarray(rowind, dtype=np.int32).ctypes.data, 4 * nnz)
-    C.memmove(A.values, np.ascontiguousarray(vals, dtype=np.float64).ctypes.data, 8 * nnz)
+    # the converted arrays are held in names until the copies are done (a
+    # temporary's buffer is freed as soon as its address has been taken)
+    cp = np.ascontiguousarray(colptr, dtype=np.int32)
+    ri = np.ascontiguousarray(rowind, dtype=np.int32)
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    C.memmove(A.colptr, cp.ctypes.data, 4 * (n + 1))
+    C.memmove(A.rowind, ri.ctypes.data, 4 * nnz)
+    C.memmove(A.values, v.ctypes.data, 8 * nnz)
     return Matrix(p)
 
 

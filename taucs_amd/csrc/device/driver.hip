@@ -22,8 +22,8 @@ template <typename T> __global__ void k_fill_ranges(const ZeroTask*, T*);
 template <typename T> __global__ void k_assemble(int, const int*, const long long*, const double*, T*);
 template <typename T> __global__ void k_front_small(const FrontDesc*, const int*, T*, T*, T*, int*);
 template <typename T> __global__ void k_potrf_diag(const FrontDesc*, const PanelTask*, T*, T*, int*);
-template <typename T> __global__ void k_potrf_wave(const FrontDesc*, const PanelTask*, T*, T*, int*);
-template <typename T> __global__ void k_front_wave(const FrontDesc*, const int*, T*, T*, T*, int*);
+template <typename T> __global__ void k_potrf_wave(const FrontDesc*, const PanelTask*, T*, T*, int*, int);
+template <typename T> __global__ void k_front_wave(const FrontDesc*, const int*, T*, T*, T*, int*, int);
 template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*, const T*);
 template <typename T> __global__ void k_trsm_tile(const GemmTask*, const T*, T*, T*, const T*);
 template <typename T> __global__ void k_gemm_tile128(const GemmTask*, const T*, T*, T*);
@@ -100,7 +100,8 @@ struct tamd_dev {
   int solve_chain = 1;
   int* d_sync = nullptr;            // the chained launches' tickets, then the error word
   long long sync_n = 0;             // tickets (cleared at the start of every sweep)
-  int* d_info = nullptr;
+  int* d_info = nullptr;            // [0] a pivot failed, [1] blocks refactored in FP64
+  int fp64_fallbacks = 0;           // of the last numeric factorization
   std::vector<Op> ops;
   std::vector<double> op_flops, op_bytes;  // algorithmic work of each op
   std::vector<float> op_ms;                // device time of each op (profiling runs)
@@ -165,6 +166,11 @@ extern "C" void taucs_amd_set_lookahead(int on) { g_lookahead = on; }
 // threads through LDS (kept for comparison).
 static int g_potrf_wave = 1;
 extern "C" void taucs_amd_set_potrf_kernel(int wave) { g_potrf_wave = wave; }
+// Reduced-precision factors: a diagonal block whose FP32 Cholesky meets a
+// nonpositive pivot or a pivot below 2^-12 of its input diagonal entry is
+// refactored in FP64 (the one-wave POTRF kernels; default on).
+static int g_fp64_fallback = 1;
+extern "C" void taucs_amd_set_fp64_fallback(int on) { g_fp64_fallback = on ? 1 : 0; }
 
 static void drop_solve_graphs(tamd_dev* D)
 {
@@ -920,7 +926,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       || upload(&D->d_ea, ea, D->stream) || upload(&D->d_sv_tasks, svt, D->stream)
       || upload(&D->d_zero, zero, D->stream)
       || upload(&D->d_colptr, colptr, D->stream)
-      || dev_malloc((void**)&D->d_info, sizeof(int))
+      || dev_malloc((void**)&D->d_info, 2 * sizeof(int))
       || dev_malloc((void**)&D->d_avals, sizeof(double) * (nnz_tot > 0 ? nnz_tot : 1)))
     return -1;
   if (hipStreamSynchronize(D->stream) != hipSuccess) return -1;
@@ -1144,7 +1150,7 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
     D->op_ms.assign(D->ops.size(), 0.0f);
   }
   EMIT(0, true, k_fill<T>, dim3(grid_for(D->L_elems, 256)), dim3(256), L, D->L_elems);
-  EMIT(0, true, k_fill<int>, dim3(1), dim3(64), D->d_info, 1LL);
+  EMIT(0, true, k_fill<int>, dim3(1), dim3(64), D->d_info, 2LL);
   // Side-lane ops (the lookahead's trailing update) may overlap the next
   // panel's POTRF/TRSM, which touch other columns; every other op joins.
   const bool la = !prof && g_lookahead && D->side;
@@ -1164,7 +1170,7 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
       case TAMD_OP_SMALL:
         if (g_potrf_wave)
           EMIT(lane, join, k_front_wave<T>, dim3(op.count), dim3(64), (const FrontDesc*)D->d_fronts,
-               (const int*)(D->d_small + op.start), L, U, Iv, D->d_info);
+               (const int*)(D->d_small + op.start), L, U, Iv, D->d_info, g_fp64_fallback);
         else
           EMIT(lane, join, k_front_small<T>, dim3(op.count), dim3(256), (const FrontDesc*)D->d_fronts,
                (const int*)(D->d_small + op.start), L, U, Iv, D->d_info);
@@ -1172,7 +1178,7 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
       case TAMD_OP_POTRF:
         if (g_potrf_wave)
           EMIT(lane, join, k_potrf_wave<T>, dim3(op.count), dim3(64), (const FrontDesc*)D->d_fronts,
-               (const PanelTask*)(D->d_potrf + op.start), L, Iv, D->d_info);
+               (const PanelTask*)(D->d_potrf + op.start), L, Iv, D->d_info, g_fp64_fallback);
         else
           EMIT(lane, join, k_potrf_diag<T>, dim3(op.count), dim3(256), (const FrontDesc*)D->d_fronts,
                (const PanelTask*)(D->d_potrf + op.start), L, Iv, D->d_info);
@@ -1264,7 +1270,7 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
     HIPCHECK(hipEventRecord(e0, st));
   }
   const int key = (int)sizeof(T) * 2 + (bf16upd ? 1 : 0) + (g_lookahead ? 16 : 0)
-                  + (g_potrf_wave ? 32 : 0);
+                  + (g_potrf_wave ? 32 : 0) + (g_fp64_fallback ? 64 : 0);
   if (prof || !g_graphs) {
     Emitter E = eager_emitter(D);
     if (launch_numeric<T>(D, E, bf16upd, prof) != 0) return -1;
@@ -1302,10 +1308,11 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
   }
   D->timing.gemm_flops = D->gemm_flops;
   D->timing.assembly_bytes = D->assembly_bytes;
-  int info = 0;
-  HIPCHECK(hipMemcpyAsync(&info, D->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+  int info[2] = {0, 0};
+  HIPCHECK(hipMemcpyAsync(info, D->d_info, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
-  if (info) {
+  D->fp64_fallbacks = info[1];
+  if (info[0]) {
     taucs_printf((char*)"\t\tLL^T Factorization: Matrix is not positive definite.\n");
     return -1;
   }
@@ -1394,6 +1401,13 @@ extern "C" int taucs_amd_get_op_profile(void* vL, int cap, int* kind, int* level
     if (ms) ms[o] = o < (int)D->op_ms.size() ? D->op_ms[o] : 0.0;
   }
   return n;
+}
+
+extern "C" int taucs_amd_factor_fp64_fallbacks(void* vL)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  if (!F || !F->dev) return -1;
+  return F->dev->fp64_fallbacks;
 }
 
 extern "C" int taucs_amd_get_numeric_timing(void* vL, taucs_amd_numeric_timing* t)

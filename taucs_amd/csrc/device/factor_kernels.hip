@@ -266,14 +266,24 @@ __device__ __forceinline__ void wave_lds_fence()
   __builtin_amdgcn_wave_barrier();
 }
 
-template <typename T, bool FRONT>
-__device__ __forceinline__ void wave_chol_inv(T (&r)[64], int lane, int nsteps, T* col, T* Ls,
-                                              T* rdiag, bool& bad)
+// Pivot growth that flags a reduced-precision diagonal block: a pivot below
+// 2^-12 of the block's input diagonal entry has lost at least 12 of FP32's 24
+// bits to cancellation.  Such a block is refactored in FP64.
+#define GROW_TAU (1.0f / 4096.0f)
+
+// Pivot k is eliminated when bit k of `piv` is set (wave-uniform): all 64 for
+// a diagonal block, the first s columns of each front for whole fronts.
+template <typename T>
+__device__ __forceinline__ void wave_chol_inv(T (&r)[64], int lane, unsigned long long piv, T* col,
+                                              T* Ls, T* rdiag, const T* dg, bool& bad, bool& grow)
 {
 #pragma clang loop unroll(full)
   for (int k = 0; k < 64; k++) {
-    if (!FRONT || k < nsteps) {
+    if ((piv >> k) & 1ULL) {
       T p = rlane(r[k], k);
+      if constexpr (sizeof(T) == 4) {
+        if (p < (T)GROW_TAU * dg[k]) grow = true;
+      }
       if (!(p > (T)0)) { bad = true; p = (T)1; }
       T dk, rdk;
       sqrt_pair(p, dk, rdk);
@@ -377,16 +387,62 @@ __device__ __forceinline__ void wave_inverse(T (&r)[64], int lane, T* Ls, const 
   __builtin_amdgcn_wave_barrier();
 }
 
+// FP64 fallback of a flagged FP32 diagonal block.  The block's input (not yet
+// overwritten in global memory) is reloaded as FP64 into the lower triangle
+// packed by rows in LDS (row i at i(i+1)/2: 2080 doubles, the footprint of the
+// FP32 staging buffer Ls), factored there for its first ns pivots by a
+// right-looking Cholesky (lane i owns row i; the column of pivot k is scaled,
+// then each lane updates its own row), and rounded back into the FP32
+// registers and reciprocal diagonal that the inverse and the stores read.
+// Rows and columns >= ns keep the FP64 Schur complement.
+__device__ __forceinline__ int ptri(int i, int j) { return ((i * (i + 1)) >> 1) + j; }
+
+__device__ __forceinline__ void wave_chol_packed_f64(double* P, int lane, unsigned long long piv,
+                                                     bool& bad)
+{
+  for (int k = 0; k < 64; k++) {
+    if (!((piv >> k) & 1ULL)) continue;
+    wave_lds_fence();
+    double p = P[ptri(k, k)];
+    if (!(p > 0.0)) { bad = true; p = 1.0; }
+    const double dk = sqrt(p), rdk = 1.0 / dk;
+    double lik = 0.0;
+    if (lane > k) {
+      lik = P[ptri(lane, k)] * rdk;
+      P[ptri(lane, k)] = lik;
+    }
+    wave_lds_fence();
+    if (lane == k) P[ptri(k, k)] = dk;
+    if (lane > k)
+      for (int j = k + 1; j <= lane; j++) P[ptri(lane, j)] -= lik * P[ptri(j, k)];
+  }
+  wave_lds_fence();
+}
+
+template <typename T>
+__device__ __forceinline__ void fp64_refactor(T (&r)[64], int lane, unsigned long long piv, T* Ls,
+                                              T* rdiag, bool& bad)
+{
+  double* P = reinterpret_cast<double*>(Ls);
+  bad = false;
+  wave_chol_packed_f64(P, lane, piv, bad);
+#pragma unroll
+  for (int c = 0; c < 64; c++) r[c] = c <= lane ? (T)P[ptri(lane, c)] : (T)0;
+  rdiag[lane] = ((piv >> lane) & 1ULL) ? (T)(1.0 / P[ptri(lane, lane)]) : (T)1;
+  wave_lds_fence();
+}
+
 template <typename T>
 __global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__ fr,
                                                    const PanelTask* __restrict__ tasks,
                                                    T* __restrict__ L, T* __restrict__ Inv,
-                                                   int* __restrict__ info)
+                                                   int* __restrict__ info, int fallback)
 {
   __shared__ __align__(16) T col[64];
   __shared__ __align__(16) T Ls[64 * PW_LD];
   __shared__ __align__(16) T rdiag[64];
   __shared__ __align__(16) T Ts[16 * 17];
+  __shared__ __align__(16) T dg[64];
   // the panel chain is the critical path: its waves win issue arbitration
   // against the trailing-update waves that share their SIMD
   __builtin_amdgcn_s_setprio(3);
@@ -399,8 +455,27 @@ __global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__
 #pragma unroll
   for (int c = 0; c < 64; c++)
     r[c] = (lane < kb && c < kb) ? gp[lane + c * m] : (lane == c ? (T)1 : (T)0);
-  bool bad = false;
-  wave_chol_inv<T, false>(r, lane, 64, col, Ls, rdiag, bad);
+  if constexpr (sizeof(T) == 4) {
+    T v = (T)1;
+#pragma unroll
+    for (int c = 0; c < 64; c++)
+      if (c == lane) v = r[c];
+    dg[lane] = v;
+    wave_lds_fence();
+  }
+  bool bad = false, grow = false;
+  wave_chol_inv<T>(r, lane, ~0ULL, col, Ls, rdiag, dg, bad, grow);
+  if constexpr (sizeof(T) == 4) {
+    if (fallback && (bad || grow)) {  // wave-uniform (pivots come by readlane)
+      double* P = reinterpret_cast<double*>(Ls);
+      const T* g0 = gp;
+      asm volatile("" : "+s"(g0));
+      for (int c = 0; c <= lane; c++)
+        P[ptri(lane, c)] = (lane < kb && c < kb) ? (double)g0[lane + c * m] : (lane == c ? 1.0 : 0.0);
+      fp64_refactor<T>(r, lane, ~0ULL, Ls, rdiag, bad);
+      if (lane == 0) atomicAdd(info + 1, 1);
+    }
+  }
   T* gw = gp;  // a fresh copy, so the loads' addresses need not stay live
   asm volatile("" : "+s"(gw));
 #pragma unroll
@@ -425,12 +500,14 @@ template <typename T>
 __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__ fr,
                                                    const int* __restrict__ list,
                                                    T* __restrict__ L, T* __restrict__ U,
-                                                   T* __restrict__ Inv, int* __restrict__ info)
+                                                   T* __restrict__ Inv, int* __restrict__ info,
+                                                   int fallback)
 {
   __shared__ __align__(16) T col[64];
   __shared__ __align__(16) T Ls[64 * PW_LD];
   __shared__ __align__(16) T rdiag[64];
   __shared__ __align__(16) T Ts[16 * 17];
+  __shared__ __align__(16) T dg[64];
   const FrontDesc d = fr[list[blockIdx.x]];
   const int s = d.s, m = d.m, u = m - s, lane = threadIdx.x;
   T* gp = L + d.offL;
@@ -446,8 +523,36 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
     }
     r[c] = v;
   }
-  bool bad = false;
-  wave_chol_inv<T, true>(r, lane, s, col, Ls, rdiag, bad);
+  if constexpr (sizeof(T) == 4) {
+    T v = (T)1;
+#pragma unroll
+    for (int c = 0; c < 64; c++)
+      if (c == lane) v = r[c];
+    dg[lane] = v;
+    wave_lds_fence();
+  }
+  bool bad = false, grow = false;
+  wave_chol_inv<T>(r, lane, s >= 64 ? ~0ULL : (1ULL << s) - 1, col, Ls, rdiag, dg, bad, grow);
+  if constexpr (sizeof(T) == 4) {
+    if (fallback && (bad || grow)) {  // wave-uniform (pivots come by readlane)
+      double* P = reinterpret_cast<double*>(Ls);
+      const T* g0 = gp;
+      asm volatile("" : "+s"(g0));
+      const T* g1 = gu;
+      asm volatile("" : "+s"(g1));
+      for (int c = 0; c <= lane; c++) {
+        double v = lane == c ? 1.0 : 0.0;
+        if (c < s) {
+          if (lane < m) v = (double)g0[lane + c * m];
+        } else if (c < m) {
+          if (lane < m) v = (double)g1[(lane - s) + (c - s) * u];
+        }
+        P[ptri(lane, c)] = v;
+      }
+      fp64_refactor<T>(r, lane, s >= 64 ? ~0ULL : (1ULL << s) - 1, Ls, rdiag, bad);
+      if (lane == 0) atomicAdd(info + 1, 1);
+    }
+  }
   T* gw = gp;
   asm volatile("" : "+s"(gw));
   T* guw = gu;
@@ -476,6 +581,268 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
 #pragma unroll
   for (int i = 0; i < 64; i++) iv[i * 64 + ln] = (ln < s && i < s) ? r[i] : (T)0;
   if (lane == 0 && bad) atomicOr(info, 1);
+}
+
+// Small fronts packed into one wavefront: fronts with m <= 16 four to a wave,
+// m <= 32 two to a wave, at lane offsets 0/16/32/48 (0/32).  The wave factors
+// the block-diagonal matrix of its fronts: pivot k is eliminated only in the
+// first s columns of its slot's front, and the couplings between slots are
+// zero, so every rank-1 update of another slot's pivot leaves a front's
+// entries exactly unchanged, and the block inverse of the 16 x 16 blocks
+// treats each slot as it treats a front at lane 0.  A front's arithmetic is
+// therefore the same in every slot (results are bitwise those of one front
+// per wave, width 64).  Per front, as k_front_wave: L in the panel columns,
+// the Schur complement in the update block, the inverse of the leading s x s
+// block (its s x s part: the solve reads nothing beyond it).
+template <typename T>
+__global__ __launch_bounds__(64) void k_front_pack(const FrontDesc* __restrict__ fr,
+                                                   const SmallPack* __restrict__ packs,
+                                                   T* __restrict__ L, T* __restrict__ U,
+                                                   T* __restrict__ Inv, int* __restrict__ info,
+                                                   int fallback)
+{
+  __shared__ __align__(16) T col[64];
+  __shared__ __align__(16) T Ls[64 * PW_LD];
+  __shared__ __align__(16) T rdiag[64];
+  __shared__ __align__(16) T Ts[16 * 17];
+  __shared__ __align__(16) T dg[64];
+  const SmallPack* pk = packs + blockIdx.x;
+  const int lane = threadIdx.x, W = pk->width;
+  const int slot = lane / W, base = slot * W, li = lane - base;
+  const int fi = pk->f[slot];
+  int s = 0, m = 0;
+  long long offL = 0, offU = 0, offI = 0;
+  if (fi >= 0) {
+    const FrontDesc d = fr[fi];
+    s = d.s;
+    m = d.m;
+    offL = d.offL;
+    offU = d.offU;
+    offI = d.offI;
+  }
+  const int u = m - s;
+  unsigned long long piv = 0;  // the first s columns of every slot's front
+  for (int q = 0; q * W < 64; q++) {
+    const int f = pk->f[q];
+    if (f >= 0) {
+      const int sq = fr[f].s;
+      piv |= (sq >= 64 ? ~0ULL : (1ULL << sq) - 1) << (q * W);
+    }
+  }
+  T* const gp = L + offL;
+  T* const gu = U + offU;
+  // row `lane` of the wave's block-diagonal matrix, staged through LDS (each
+  // lane's columns are those of its own slot, at run-time offsets)
+  T* row = Ls + lane * PW_LD;
+#pragma unroll
+  for (int c = 0; c < 64; c++) row[c] = c == lane ? (T)1 : (T)0;
+  if (fi >= 0 && li < m) {
+    for (int cl = 0; cl <= li; cl++) {
+      if (cl < s) row[base + cl] = gp[li + (long long)cl * m];
+      else row[base + cl] = gu[(li - s) + (long long)(cl - s) * u];
+    }
+  }
+  wave_lds_fence();
+  T r[64];
+#pragma unroll
+  for (int c = 0; c < 64; c++) r[c] = row[c];
+  if constexpr (sizeof(T) == 4) {
+    dg[lane] = row[lane];
+    wave_lds_fence();
+  }
+  bool bad = false, grow = false;
+  wave_chol_inv<T>(r, lane, piv, col, Ls, rdiag, dg, bad, grow);
+  if constexpr (sizeof(T) == 4) {
+    if (fallback && (bad || grow)) {  // wave-uniform: every front of the wave in FP64
+      double* P = reinterpret_cast<double*>(Ls);
+      for (int c = 0; c <= lane; c++) P[ptri(lane, c)] = c == lane ? 1.0 : 0.0;
+      if (fi >= 0 && li < m) {
+        for (int cl = 0; cl <= li; cl++) {
+          const int c = base + cl;
+          if (cl < s) P[ptri(lane, c)] = (double)gp[li + (long long)cl * m];
+          else P[ptri(lane, c)] = (double)gu[(li - s) + (long long)(cl - s) * u];
+        }
+      }
+      fp64_refactor<T>(r, lane, piv, Ls, rdiag, bad);
+      if (lane == 0) atomicAdd(info + 1, 1);
+    }
+  }
+  // the factored entries back through LDS: L in the panel columns, the Schur
+  // complement in the update block
+  wave_lds_fence();
+#pragma unroll
+  for (int c = 0; c < 64; c++) row[c] = r[c];
+  wave_lds_fence();
+  if (fi >= 0 && li < m) {
+    for (int cl = 0; cl <= li; cl++) {
+      if (cl < s) gp[li + (long long)cl * m] = row[base + cl];
+      else gu[(li - s) + (long long)(cl - s) * u] = row[base + cl];
+    }
+  }
+  // the inverse of each front's leading s x s block: its other rows become the
+  // identity (they hold the Schur complement, not L)
+  if (li >= s) {
+#pragma unroll
+    for (int c = 0; c < 64; c++) r[c] = c == lane ? (T)1 : (T)0;
+    rdiag[lane] = (T)1;
+  }
+  wave_lds_fence();
+  wave_inverse<T>(r, lane, Ls, rdiag, Ts);
+  // lane holds column `lane` of the inverse: X(base + i, lane) = r[base + i]
+#pragma unroll
+  for (int c = 0; c < 64; c++) row[c] = r[c];
+  wave_lds_fence();
+  if (fi >= 0 && li < s) {
+    T* iv = Inv + offI;
+    for (int i = 0; i < s; i++) iv[i * 64 + li] = row[base + i];
+  }
+  if (lane == 0 && bad) atomicOr(info, 1);
+}
+
+// Fronts with m <= 16, four to a wavefront, lane = row within its slot of 16
+// lanes and each lane holding only its slot's 16 columns (a fifth of the
+// registers of the 64-wide kernels, so several waves share a SIMD).  Every
+// operation on a front's entries is the one k_front_wave performs on a front
+// at lane 0 (pivot by lane broadcast, sqrt_pair, rank-1 update through LDS,
+// the 16 x 16 diagonal-block substitution of wave_inverse), so the results
+// are bitwise those of one front per wave.
+template <typename T>
+__global__ __launch_bounds__(64) void k_front_slot16(const FrontDesc* __restrict__ fr,
+                                                     const SmallPack* __restrict__ packs,
+                                                     T* __restrict__ L, T* __restrict__ U,
+                                                     T* __restrict__ Inv, int* __restrict__ info,
+                                                     int fallback)
+{
+  constexpr int NSB = sizeof(T) == 4 ? 2080 * 8 : 64 * 17 * 8;  // FP32: room for the FP64 fallback
+  __shared__ __align__(16) unsigned char sbuf[NSB];
+  __shared__ __align__(16) T col[64];
+  __shared__ __align__(16) T rdiag[64];
+  __shared__ __align__(16) T dg[64];
+  T* Ls = reinterpret_cast<T*>(sbuf);  // row `lane`: the 16 columns of its slot (ld 17)
+  const SmallPack* pk = packs + blockIdx.x;
+  const int lane = threadIdx.x, slot = lane >> 4, base = slot << 4, li = lane & 15;
+  const int fi = pk->f[slot];
+  int s = 0, m = 0;
+  long long offL = 0, offU = 0, offI = 0;
+  if (fi >= 0) {
+    const FrontDesc d = fr[fi];
+    s = d.s;
+    m = d.m;
+    offL = d.offL;
+    offU = d.offU;
+    offI = d.offI;
+  }
+  const int u = m - s;
+  T* const gp = L + offL;
+  T* const gu = U + offU;
+  T r[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    T v = li == c ? (T)1 : (T)0;
+    if (fi >= 0 && li < m) {
+      if (c < s) v = gp[li + (long long)c * m];
+      else if (c < m && li >= s) v = gu[(li - s) + (long long)(c - s) * u];
+    }
+    r[c] = v;
+  }
+  if constexpr (sizeof(T) == 4) {
+    T v = (T)1;
+#pragma unroll
+    for (int c = 0; c < 16; c++)
+      if (c == li) v = r[c];
+    dg[lane] = v;
+    wave_lds_fence();
+  }
+  bool bad = false, grow = false;
+#pragma clang loop unroll(full)
+  for (int k = 0; k < 16; k++) {
+    const T p0 = rlane(r[k], k), p1 = rlane(r[k], 16 + k), p2 = rlane(r[k], 32 + k),
+            p3 = rlane(r[k], 48 + k);
+    T p = slot == 0 ? p0 : slot == 1 ? p1 : slot == 2 ? p2 : p3;
+    const bool act = k < s;
+    if (act) {
+      if constexpr (sizeof(T) == 4) {
+        if (p < (T)GROW_TAU * dg[base + k]) grow = true;
+      }
+      if (!(p > (T)0)) { bad = true; p = (T)1; }
+    }
+    T dk, rdk;
+    sqrt_pair(p, dk, rdk);
+    const T lik = act ? r[k] * rdk : (T)0;
+    if (act) {
+      r[k] = li == k ? dk : lik;
+      if (li == k) rdiag[lane] = rdk;
+    }
+    col[lane] = r[k];
+    wave_lds_fence();
+#pragma clang loop unroll(full)
+    for (int j = k + 1; j < 16; j++) r[j] -= lik * col[base + j];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if constexpr (sizeof(T) == 4) {
+    // a flagged slot anywhere in the wave: every front of the wave in FP64
+    if (fallback && __builtin_amdgcn_read_exec() && __any_sync(~0ULL, bad || grow)) {
+      double* P = reinterpret_cast<double*>(sbuf);
+      unsigned long long piv = 0;
+      for (int q = 0; q < 4; q++) {
+        const int f = pk->f[q];
+        if (f >= 0) piv |= ((1ULL << fr[f].s) - 1) << (16 * q);
+      }
+      for (int c = 0; c <= lane; c++) P[ptri(lane, c)] = c == lane ? 1.0 : 0.0;
+      if (fi >= 0 && li < m) {
+        for (int cl = 0; cl <= li; cl++) {
+          if (cl < s) P[ptri(lane, base + cl)] = (double)gp[li + (long long)cl * m];
+          else P[ptri(lane, base + cl)] = (double)gu[(li - s) + (long long)(cl - s) * u];
+        }
+      }
+      bool bad64 = false;
+      wave_chol_packed_f64(P, lane, piv, bad64);
+#pragma unroll
+      for (int c = 0; c < 16; c++) r[c] = c <= li ? (T)P[ptri(lane, base + c)] : (T)0;
+      rdiag[lane] = li < s ? (T)(1.0 / P[ptri(lane, lane)]) : (T)1;
+      wave_lds_fence();
+      bad = bad64;
+      if (lane == 0) atomicAdd(info + 1, 1);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    if (fi >= 0 && li < m && li >= c) {
+      if (c < s) gp[li + (long long)c * m] = r[c];
+      else if (c < m) gu[(li - s) + (long long)(c - s) * u] = r[c];
+    }
+  }
+  // inverse of each front's leading s x s block (its other rows: identity)
+  if (li >= s) {
+#pragma unroll
+    for (int c = 0; c < 16; c++) r[c] = c == li ? (T)1 : (T)0;
+    rdiag[lane] = (T)1;
+  }
+  wave_lds_fence();
+#pragma unroll
+  for (int c = 0; c < 16; c++) Ls[lane * 17 + c] = r[c];
+  wave_lds_fence();
+  const T* Lb = Ls + base * 17;
+  T x[16];
+#pragma clang loop unroll(full)
+  for (int i = 0; i < 16; i++) {
+    T a0 = (i == li) ? (T)1 : (T)0, a1 = (T)0;
+#pragma clang loop unroll(full)
+    for (int t = 0; t < i; t += 2) {
+      a0 -= Lb[i * 17 + t] * x[t];
+      if (t + 1 < i) a1 -= Lb[i * 17 + t + 1] * x[t + 1];
+    }
+    x[i] = (a0 + a1) * rdiag[base + i];
+    __builtin_amdgcn_sched_barrier(0);  // one row's loads at a time (registers)
+  }
+  if (fi >= 0 && li < s) {
+    T* iv = Inv + offI;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+      if (i < s) iv[i * 64 + li] = i >= li ? x[i] : (T)0;
+  }
+  if (bad) atomicOr(info, 1);
 }
 
 // ---- MFMA tile GEMM ---------------------------------------------------------
@@ -685,6 +1052,98 @@ __global__ __launch_bounds__(256) void k_gemm_tile128(const GemmTask* __restrict
       }
 }
 
+// 128 x 128 tile without LDS: each wave reads its own MFMA operands straight
+// from global memory (lane (c16, g) of k-step s: row c16 of each 16-row block
+// at k = 4s + g, the layout the 16x16x4 MFMA takes), two k-steps ahead in
+// registers, so there is no barrier and no LDS traffic; the two waves that
+// share a row (column) block find its lines in L1.  Rows beyond M and columns
+// beyond N are clamped to the last valid one (their results are not stored);
+// a K tail is zero-filled.  The k order of every sum is that of
+// k_gemm_tile128, so the two kernels give bitwise equal results.
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_gemm_tile128d(const GemmTask* __restrict__ tasks,
+                                                         const T* Lsrc, T* L, T* U)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  const GemmTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int c16 = lane & 15, g = lane >> 4;
+  const long long ld = t.ld;
+  const T* pa[4];
+  const T* pb[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const int ia = min(wr + 16 * a + c16, t.M - 1), jb = min(wc + 16 * a + c16, t.N - 1);
+    pa[a] = Lsrc + t.a_off + ia + (long long)g * ld;
+    pb[a] = Lsrc + t.b_off + jb + (long long)g * ld;
+  }
+  acc_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
+  const int K = t.K, nfull = K >> 2;
+  const long long step = 4 * ld;
+  T fa[3][4], fb[3][4];  // a ring of three k-steps of operands
+  auto load = [&](int slot, int ks) {
+    if (ks < nfull) {
+      const long long o = (long long)ks * step;
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        fa[slot][a] = pa[a][o];
+        fb[slot][a] = pb[a][o];
+      }
+    }
+  };
+  auto mma = [&](int slot, int ks) {
+    if (ks < nfull) {
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::step(fb[slot][b], fa[slot][a], acc[a][b]);
+    }
+  };
+  load(0, 0);
+  load(1, 1);
+  for (int ks = 0; ks < nfull; ks += 3) {
+    load(2, ks + 2);
+    mma(0, ks);
+    load(0, ks + 3);
+    mma(1, ks + 1);
+    load(1, ks + 4);
+    mma(2, ks + 2);
+  }
+  if (K & 3) {  // tail: k = 4 nfull + g exists only for g < K mod 4
+    const bool kin = g < (K & 3);
+    const long long o = (long long)nfull * step;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      fa[0][a] = kin ? pa[a][o] : (T)0;
+      fb[0][a] = kin ? pb[a][o] : (T)0;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::step(fb[0][b], fa[0][a], acc[a][b]);
+  }
+
+  T* C = ((t.flags & 1) ? U : L) + t.c_off;
+  const bool diag = (t.flags & 2) != 0;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = wr + a * 16 + (lane & 15);
+        const int j = wc + b * 16 + Mfma<T>::drow(lane, r);
+        if (i < t.M && j < t.N && (!diag || i >= j)) C[i + (long long)j * t.ldc] -= acc[a][b][r];
+      }
+}
+
 // Same tile, BF16 inputs with FP32 accumulation (v_mfma_f32_16x16x32_bf16):
 // the update variant of the FP32 factor.  Operands are rounded to bf16 as
 // they are staged into LDS; K chunk 32.
@@ -835,11 +1294,14 @@ __global__ __launch_bounds__(256) void k_extend_add_seq(const FrontDesc* __restr
                                          const double*, T*);                            \
   template __global__ void k_front_small<T>(const FrontDesc*, const int*, T*, T*, T*, int*); \
   template __global__ void k_potrf_diag<T>(const FrontDesc*, const PanelTask*, T*, T*, int*); \
-  template __global__ void k_potrf_wave<T>(const FrontDesc*, const PanelTask*, T*, T*, int*); \
-  template __global__ void k_front_wave<T>(const FrontDesc*, const int*, T*, T*, T*, int*); \
+  template __global__ void k_potrf_wave<T>(const FrontDesc*, const PanelTask*, T*, T*, int*, int); \
+  template __global__ void k_front_wave<T>(const FrontDesc*, const int*, T*, T*, T*, int*, int); \
+  template __global__ void k_front_pack<T>(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int); \
+  template __global__ void k_front_slot16<T>(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int); \
   template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_trsm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_gemm_tile128<T>(const GemmTask*, const T*, T*, T*);          \
+  template __global__ void k_gemm_tile128d<T>(const GemmTask*, const T*, T*, T*);         \
   template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*); \
   template __global__ void k_extend_add_seq<T>(const FrontDesc*, const EATask*, const EATask*,   \
                                                const int*, T*, T*);

@@ -49,6 +49,12 @@ struct PanelTask {  // one diagonal block, or one row block of the panel
   int r0, nr;       // rows (or columns) r0 .. r0+nr-1
 };
 
+struct SmallPack {  // fronts with m <= width, one per `width`-lane slot of a wavefront
+  int f[4];         // front indices (-1: empty slot)
+  int width;        // 16, 32 or 64
+  int pad[3];
+};
+
 struct ZeroTask {   // U[off .. off+len) <- 0
   long long off, len;
 };

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_fwd_step(const FrontDesc* __restrict__ 
 
 // L21 rows of the forward solve, split over column chunks: the task's rows
 // r0..r0+nr-1 (>= s) times y of columns [c0, c0+nc) (k0 = c0, kb = nc) into
-// the partial block Part[slot] (64 x SMAXR); k_fwd_rect_reduce subtracts the
+// the partial block Part[slot] (64 x nrhs); k_fwd_rect_reduce subtracts the
 // partial blocks from W_f in slot order, so the sum is deterministic.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_rect(const FrontDesc* __restrict__ fr,
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void k_fwd_rect(const FrontDesc* __restrict__ 
       acc[q] += a;
     }
   }
-  T* part = Part + (long long)blockIdx.x * (64 * SMAXR);
+  T* part = Part + (long long)blockIdx.x * (64LL * nrhs);
   for (int q = 0; q < 4; q++) {
     int r = 4 * g + q;
     if (r >= nrhs) break;
@@ -205,7 +205,7 @@ __global__ void k_fwd_rect_reduce(const FrontDesc* __restrict__ fr,
     int i = e & 63, r = e >> 6;
     if (i >= t.nr) continue;
     T sum = (T)0;
-    for (int q = 0; q < t.kb; q++) sum += Part[(long long)(t.k0 + q) * (64 * SMAXR) + i + r * 64];
+    for (int q = 0; q < t.kb; q++) sum += Part[(long long)(t.k0 + q) * (64LL * nrhs) + i + r * 64];
     w[t.r0 + i + r * d.m] -= sum;
   }
 }
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(256) void k_bwd_cols(const FrontDesc* __restrict__ 
       acc[q] += a;
     }
   }
-  T* part = Part + (long long)blockIdx.x * (64 * SMAXR);
+  T* part = Part + (long long)blockIdx.x * (64LL * nrhs);
   for (int q = 0; q < 4; q++) {
     int r = 4 * g + q;
     if (r >= nrhs) break;
@@ -340,7 +340,7 @@ __global__ void k_bwd_cols_reduce(const FrontDesc* __restrict__ fr,
     int j = e & 63, r = e >> 6;
     if (j >= t.nr) continue;
     T sum = (T)0;
-    for (int q = 0; q < t.kb; q++) sum += Part[(long long)(t.k0 + q) * (64 * SMAXR) + j + r * 64];
+    for (int q = 0; q < t.kb; q++) sum += Part[(long long)(t.k0 + q) * (64LL * nrhs) + j + r * 64];
     Y[rows[d.rows + t.r0 + j] + (long long)r * ldy] -= sum;
   }
 }
@@ -666,16 +666,20 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const FrontDesc* __restrict__
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int q = 16 * wv + Mfma<T>::drow(lane, k);
-    const T* ps = Part + (long long)t.kb * (64 * SMAXR) + q + c16 * 64;
+    // slot stride 64 x nrhs: only the columns c16 < nrhs exist
+    const long long pst = 64LL * nrhs;
+    const T* ps = Part + (long long)t.kb * pst + q + c16 * 64;
     T s0 = (T)0, s1 = (T)0, s2 = (T)0, s3 = (T)0;  // slots in four interleaved sums
-    int u = 0;
-    for (; u + 3 < t.nr; u += 4) {
-      s0 += ps[(long long)u * (64 * SMAXR)];
-      s1 += ps[(long long)(u + 1) * (64 * SMAXR)];
-      s2 += ps[(long long)(u + 2) * (64 * SMAXR)];
-      s3 += ps[(long long)(u + 3) * (64 * SMAXR)];
+    if (c16 < nrhs) {
+      int u = 0;
+      for (; u + 3 < t.nr; u += 4) {
+        s0 += ps[(long long)u * pst];
+        s1 += ps[(long long)(u + 1) * pst];
+        s2 += ps[(long long)(u + 2) * pst];
+        s3 += ps[(long long)(u + 3) * pst];
+      }
+      for (; u < t.nr; u++) s0 += ps[(long long)u * pst];
     }
-    for (; u < t.nr; u++) s0 += ps[(long long)u * (64 * SMAXR)];
     const T sum = (s0 + s1) + (s2 + s3);
     y0[k] = (c16 < nrhs && q < kb) ? Y[rw[c0 + q] + (long long)c16 * ldy] - sum : (T)0;
   }
@@ -785,9 +789,12 @@ __global__ __launch_bounds__(256) void k_bwd_upd(const FrontDesc* __restrict__ f
 #pragma unroll
   for (int st = 0; st < BU_ROWS / 4; st++)
     if (st < nst) D = Mfma<T>::step(a[st], Xl[(4 * st + kq) * YS + c16], D);
-  T* part = Part + (long long)blockIdx.x * (64 * SMAXR);
+  // slot stride 64 x nrhs: the columns c16 >= nrhs are not stored
+  T* part = Part + (long long)blockIdx.x * (64LL * nrhs);
+  if (c16 < nrhs) {
 #pragma unroll
-  for (int k = 0; k < 4; k++) part[16 * wv + Mfma<T>::drow(lane, k) + c16 * 64] = D[k];
+    for (int k = 0; k < 4; k++) part[16 * wv + Mfma<T>::drow(lane, k) + c16 * 64] = D[k];
+  }
 }
 
 // Fronts with m <= 64, one wave each (four per workgroup), no waits: the

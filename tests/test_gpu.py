@@ -230,17 +230,29 @@ def test_linsolve_protocol():
 def test_linsolve_cg_stops_on_the_reference_rule():
     """taucs.solve.cg=true runs CG preconditioned by the factor and stops on
     |r|_2 <= convergetol |b|_2 (src/taucs_iter.c:139,186), for the FP64 and a
-    reduced-precision factor."""
+    reduced-precision factor.  The operator (conductivities over four decades,
+    shift 1e-6) has a relative-residual floor near 1e-10 in FP64 (the direct
+    solve's, printed), so the tolerances stay above it: a tolerance below the
+    floor cannot be met by any FP64 iteration, and CG then returns its last
+    iterate with success, as the reference's does."""
     print("case: linsolve cg convergetol", flush=True)
     A = T.mesh3d_varcoef(16, 16, 16, seed=11)
     n = A.n
     B = np.stack([T.random_uniform(n, 40 + r) for r in range(2)], 1)
     M = A.to_scipy_full()
-    for prec, tol in (("fp64", 1e-13), ("fp32", 1e-10), ("fp32", 1e-13)):
-        rc, Y, _ = T.linsolve(A, B, ["taucs.factor.LLT=true", "taucs.factor.mf=true",
-                                     f"taucs.factor.precision={prec}", "taucs.solve.cg=true",
-                                     f"taucs.solve.convergetol={tol}"])
-        rr = np.linalg.norm(B - M @ Y, axis=0) / np.linalg.norm(B, axis=0)
+
+    def rres(Y):
+        return np.linalg.norm(B - M @ Y, axis=0) / np.linalg.norm(B, axis=0)
+
+    base = ["taucs.factor.LLT=true", "taucs.factor.mf=true"]
+    rc, Y, _ = T.linsolve(A, B, base)
+    assert rc == T.TAUCS_SUCCESS
+    print(f"  direct fp64 solve relres {rres(Y).max():.2e}", flush=True)
+    for prec, tol in (("fp64", 1e-8), ("fp32", 1e-5), ("fp32", 1e-8)):
+        rc, Y, _ = T.linsolve(A, B, base + [f"taucs.factor.precision={prec}", "taucs.solve.cg=true",
+                                            f"taucs.solve.convergetol={tol}"])
+        rr = rres(Y)
+        print(f"  cg {prec} tol {tol:.0e}: relres {rr.max():.2e}", flush=True)
         assert rc == T.TAUCS_SUCCESS and rr.max() <= tol, (prec, tol, rr)
 
 
@@ -263,9 +275,13 @@ def test_linsolve_cg_with_a_kept_factor_on_a_new_matrix():
         A2 = T.from_arrays(n, cp, ri, v2)
         M2 = A2.to_scipy_full()
         rc, Y, _ = T.linsolve(A2, B, ["taucs.factor=false", "taucs.solve.cg=true",
-                                      "taucs.solve.convergetol=1e-12", "taucs.solve.maxits=200"], F=F)
+                                      "taucs.solve.convergetol=1e-8", "taucs.solve.maxits=200"], F=F)
         rr = np.linalg.norm(B - M2 @ Y, axis=0) / np.linalg.norm(B, axis=0)
-        assert rc == T.TAUCS_SUCCESS and rr.max() <= 1e-12, (scale, rr)
+        # against the old values the residual of the new system would stay
+        # near 0.5 (2A) or O(0.1) (D A D); 1e-8 is above this operator's FP64
+        # relative-residual floor (about 1e-10)
+        print(f"  {scale}: relres {rr.max():.2e}", flush=True)
+        assert rc == T.TAUCS_SUCCESS and rr.max() <= 1e-8, (scale, rr)
     T.linsolve(None, None, [], F=F)
 
 
@@ -286,8 +302,12 @@ def test_refinement_reads_values_changed_in_place():
     x2, i2 = F.solve_refined(PA, b, method=1, tol=1e-14, maxit=100)
     assert i2.converged
     M = PA.to_scipy_full()
-    assert np.linalg.norm(b - M @ x2) / np.linalg.norm(b) <= 1e-12
-    assert np.allclose(x2, x1 / 1.5, rtol=1e-9, atol=1e-12)
+    r = b - M @ x2
+    bwd = abs(r).max() / (abs(M).sum(1).max() * abs(x2).max() + abs(b).max())
+    # with the old values the iterate would solve A, not 1.5 A (x2 = x1);
+    # both solutions are backward stable, so they agree to about cond(A) u
+    assert bwd <= 1e-14
+    assert np.linalg.norm(x2 - x1 / 1.5) <= 1e-6 * np.linalg.norm(x1 / 1.5)
 
 
 @pytest.mark.parametrize("prec", [T.FP32])
@@ -323,3 +343,67 @@ def test_induced_device_allocation_failures_are_clean():
     F = T.factor_llt_mf(PA)
     b = T.random_uniform(A.n, 1)
     assert relres(PA, F.solve(b), b) <= 1e-12
+
+
+def _dense_lower(M):
+    """The lower triangle of a dense symmetric matrix as a TAUCS matrix with
+    every entry stored (explicit zeros too): one supernode of n columns."""
+    n = M.shape[0]
+    cp, ri, v = [0], [], []
+    for j in range(n):
+        for i in range(j, n):
+            ri.append(i)
+            v.append(M[i, j])
+        cp.append(len(ri))
+    return T.from_arrays(n, np.array(cp), np.array(ri), np.array(v))
+
+
+def _neumann_biharmonic(n, shift):
+    """D^T D + shift I, D the (n-2) x n second difference: integer entries plus
+    a power-of-two shift, so the matrix is exact in FP32; its last pivots are
+    about 1e-5 of the diagonal (condition number ~2e7)."""
+    D = np.zeros((n - 2, n))
+    for i in range(n - 2):
+        D[i, i:i + 3] = (1.0, -2.0, 1.0)
+    return D.T @ D + shift * np.eye(n)
+
+
+@pytest.mark.parametrize("n", [40, 128])
+def test_fp64_fallback_of_flagged_fp32_blocks(n):
+    """A diagonal block whose FP32 pivots lose more than 12 bits to
+    cancellation is refactored in FP64 (k_potrf_wave for the blocked front of
+    128 columns, k_front_wave for the 40-column front).  The matrix is exact
+    in FP32 and its flagged block has no FP32-rounded input (the coupling to
+    the first block is zero), so with the fallback the FP32 factor's diagonal
+    is the FP64 factor's rounded to FP32 (relative 1e-6); without it, FP32
+    Cholesky loses several digits in the last pivots (relative error > 1e-3;
+    about 6% in a CPU simulation of the same elimination order)."""
+    print(f"case: fp64 fallback, dense {n}", flush=True)
+    M = np.zeros((n, n))
+    if n > 64:
+        M[:64, :64] = 4.0 * np.eye(64)
+        M[64:, 64:] = _neumann_biharmonic(n - 64, 2.0 ** -20)
+    else:
+        M = _neumann_biharmonic(n, 2.0 ** -20)
+    A = _dense_lower(M)
+    perm, inv = T.order(A, "identity")
+    PA = T.permute(A, perm, inv)
+    d64 = T.factor_llt_mf(PA).diag()
+    try:
+        T.lib.taucs_amd_set_fp64_fallback(0)
+        F0 = T.factor_llt_mf(PA, precision=T.FP32)
+        assert T.lib.taucs_amd_factor_fp64_fallbacks(F0.h) == 0
+        e_off = (np.abs(F0.diag() - d64) / np.abs(d64)).max()
+    finally:
+        T.lib.taucs_amd_set_fp64_fallback(1)
+    F1 = T.factor_llt_mf(PA, precision=T.FP32)
+    nfb = T.lib.taucs_amd_factor_fp64_fallbacks(F1.h)
+    rel = np.abs(F1.diag() - d64) / np.abs(d64)
+    print(f"  fallbacks {nfb}; diag rel err with fallback {rel.max():.2e}, without {e_off:.2e}",
+          flush=True)
+    assert nfb == 1
+    assert rel.max() <= 1e-6
+    assert e_off > 1e-3
+    b = T.random_uniform(n, 77)
+    x, info = F1.solve_refined(PA, b, method=0, tol=1e-14, maxit=30, factored=True)
+    assert info.converged and info.backward_error <= 1e-14
