@@ -76,6 +76,20 @@ int taucs_amd_supernodal_solve_many(void* L, int nrhs, double* X, int ldx,
 int taucs_amd_supernodal_solve_many_device(void* L, int nrhs, double* dX, int ldx,
                                            const double* dB, int ldb);
 
+/* Allocation-free variant of taucs_amd_supernodal_solve_many_device.  No
+   counterpart in the reference: it is for callers that cannot allocate device
+   memory (code under graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The sweeps run eagerly on the factor's stream with their
+   workspace in one fixed device array per GPU of
+   taucs_amd_noalloc_scratch_bytes() bytes (64 MiB), held until the process
+   ends; right-hand sides go through it as many at a time as fit (at most 16).
+   Returns -1 with a message when one right-hand side's workspace exceeds the
+   array.  All calls on one GPU share the array: calls that overlap on one GPU
+   race and must be serialized by the caller. */
+int       taucs_amd_supernodal_solve_many_device_noalloc(void* L, int nrhs, double* dX, int ldx,
+                                                         const double* dB, int ldb);
+long long taucs_amd_noalloc_scratch_bytes(void);
+
 /* ---- precision and refinement ------------------------------------------- */
 #define TAUCS_AMD_FP64    0
 #define TAUCS_AMD_FP32    1

@@ -100,6 +100,9 @@ _sigs = {
                                                   C.c_void_p, C.c_int]),
     "taucs_amd_supernodal_solve_many_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                                          C.c_void_p, C.c_int]),
+    "taucs_amd_supernodal_solve_many_device_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
+                                                                 C.c_int, C.c_void_p, C.c_int]),
+    "taucs_amd_noalloc_scratch_bytes": (C.c_longlong, []),
     "taucs_amd_set_fp64_fallback": (None, [C.c_int]),
     "taucs_amd_factor_fp64_fallbacks": (C.c_int, [C.c_void_p]),
     "taucs_amd_factor_llt_mf_precision": (C.c_void_p, [_ccs_p, C.c_int]),

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libtaucs_amd.so")
 HOST_SOURCES = ["util.c", "ccs.c", "mmd.c", "symbolic.c", "order.c", "factor.c", "linsolve.c", "io.c",
                 "batch.c"]
 DEVICE_SOURCES = ["factor_kernels.hip", "solve_kernels.hip", "driver.hip", "refine.hip",
-                  "ccs_solve.hip"]
+                  "ccs_solve.hip", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
