@@ -182,6 +182,10 @@ void  taucs_amd_set_lookahead(int on);
 /* Diagonal-block POTRF kernel: 1 = one wave, register broadcasts (default);
    0 = 256 threads through LDS. */
 void  taucs_amd_set_potrf_kernel(int wave);
+/* Fronts with at most 16 (32) rows are factored four (two) to a wavefront
+   (default on); the results are bitwise those of one front per wavefront.
+   Applies to schedules built afterwards. */
+void  taucs_amd_set_front_packing(int on);
 /* Triangular sweeps of the solve: 1 = one launch per tree level, the block
    chain of each front followed inside the launch through per-block flags
    (default); 0 = one launch per 64-column block step.  Applies to schedules

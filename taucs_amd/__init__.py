@@ -118,6 +118,7 @@ _sigs = {
     "taucs_amd_set_graphs": (None, [C.c_int]),
     "taucs_amd_set_lookahead": (None, [C.c_int]),
     "taucs_amd_set_potrf_kernel": (None, [C.c_int]),
+    "taucs_amd_set_front_packing": (None, [C.c_int]),
     "taucs_amd_set_solve_kernel": (None, [C.c_int]),
     "taucs_amd_set_panel_width": (None, [C.c_int]),
     "taucs_amd_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

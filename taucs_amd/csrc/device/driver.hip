@@ -24,9 +24,12 @@ template <typename T> __global__ void k_front_small(const FrontDesc*, const int*
 template <typename T> __global__ void k_potrf_diag(const FrontDesc*, const PanelTask*, T*, T*, int*);
 template <typename T> __global__ void k_potrf_wave(const FrontDesc*, const PanelTask*, T*, T*, int*, int);
 template <typename T> __global__ void k_front_wave(const FrontDesc*, const int*, T*, T*, T*, int*, int);
+template <typename T> __global__ void k_front_slot16(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int);
+template <typename T> __global__ void k_front_slot32(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int);
 template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*, const T*);
 template <typename T> __global__ void k_trsm_tile(const GemmTask*, const T*, T*, T*, const T*);
 template <typename T> __global__ void k_gemm_tile128(const GemmTask*, const T*, T*, T*);
+template <typename T> __global__ void k_gemm_short(const GemmTask*, const T*, T*, T*, const T*);
 __global__ void k_gemm_tile_bf16(const GemmTask*, const float*, float*, float*);
 template <typename T> __global__ void k_extend_add(const FrontDesc*, const EATask*, const int*, T*, T*);
 template <typename T> __global__ void k_extend_add_seq(const FrontDesc*, const EATask*, const EATask*, const int*, T*, T*);
@@ -71,6 +74,7 @@ struct Op {
   bool nojoin = false;  // touches only the current outer panel: need not wait for the side stream
   int tile = 64;        // GEMM tile edge of the op's tasks (64 or 128)
   bool grouped = false; // extend-add: the tasks are group heads (k_extend_add_seq)
+  bool shortk = false;  // GEMM of 64-tiles whose tasks all have K <= 64 (k_gemm_short)
 };
 
 // solve op kinds
@@ -88,7 +92,8 @@ struct tamd_dev {
   int* d_rel = nullptr;
   int* d_colptr = nullptr;
   long long* d_adest = nullptr;
-  int* d_small = nullptr;        // front indices for k_front_small
+  int* d_small = nullptr;        // front indices for k_front_small / k_front_wave
+  SmallPack* d_packs = nullptr;  // fronts packed 4 (m <= 16) or 2 (m <= 32) to a wave
   PanelTask* d_potrf = nullptr;
   GemmTask* d_trsm = nullptr;   // panel solves as GEMMs with the block inverses
   GemmTask* d_gemm = nullptr;
@@ -166,6 +171,11 @@ extern "C" void taucs_amd_set_lookahead(int on) { g_lookahead = on; }
 // threads through LDS (kept for comparison).
 static int g_potrf_wave = 1;
 extern "C" void taucs_amd_set_potrf_kernel(int wave) { g_potrf_wave = wave; }
+// Small fronts several to a wavefront: m <= 16 four per wave (k_front_slot16),
+// m <= 32 two per wave (k_front_slot32); results are bitwise those of one
+// front per wave.  Applies to schedules built afterwards (default on).
+static int g_front_packing = 1;
+extern "C" void taucs_amd_set_front_packing(int on) { g_front_packing = on ? 1 : 0; }
 // Reduced-precision factors: a diagonal block whose FP32 Cholesky meets a
 // nonpositive pivot or a pivot below 2^-12 of its input diagonal entry is
 // refactored in FP64 (the one-wave POTRF kernels; default on).
@@ -281,7 +291,8 @@ extern "C" void tamd_dev_free(tamd_factor* F)
   tamd_refine_forget(F);
   tamd_dev_free_numeric(F);
   free_ptr(D->d_fronts); free_ptr(D->d_rows); free_ptr(D->d_rel); free_ptr(D->d_colptr);
-  free_ptr(D->d_adest); free_ptr(D->d_small); free_ptr(D->d_potrf); free_ptr(D->d_trsm);
+  free_ptr(D->d_adest); free_ptr(D->d_small); free_ptr(D->d_packs); free_ptr(D->d_potrf);
+  free_ptr(D->d_trsm);
   free_ptr(D->d_gemm); free_ptr(D->d_ea); free_ptr(D->d_sv_tasks); free_ptr(D->d_zero);
   free_ptr(D->d_info); free_ptr(D->d_avals); free_ptr(D->d_x); free_ptr(D->d_b);
   if (D->ev_fork) (void)hipEventDestroy(D->ev_fork);
@@ -399,6 +410,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
 
   // factorization task lists, level by level
   std::vector<int> small;
+  std::vector<SmallPack> packs;
   std::vector<PanelTask> potrf;
   std::vector<GemmTask> trsm;
   std::vector<GemmTask> gemm;
@@ -447,13 +459,34 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     int st = (int)small.size();
     int maxouter = 0;
     const int W = D->panel_w;
+    std::vector<int> c16, c32;
     for (int k = b0; k < b1; k++) {
-      if (fr[k].m <= SMALL_M) small.push_back(k);
+      if (g_front_packing && fr[k].m <= 16) c16.push_back(k);
+      else if (g_front_packing && fr[k].m <= 32) c32.push_back(k);
+      else if (fr[k].m <= SMALL_M) small.push_back(k);
       else maxouter = std::max(maxouter, (fr[k].s + W - 1) / W);
+    }
+    // own blocks only: not the parents' blocks zeroed beside them
+    for (int cls = 0; cls < 2; cls++) {
+      const std::vector<int>& cf = cls == 0 ? c16 : c32;
+      const int per = cls == 0 ? 4 : 2;
+      if (cf.empty()) continue;
+      const int p0 = (int)packs.size();
+      for (size_t i = 0; i < cf.size(); i += per) {
+        SmallPack pk;
+        for (int q = 0; q < 4; q++) pk.f[q] = (q < per && i + q < cf.size()) ? cf[i + q] : -1;
+        pk.width = cls == 0 ? 16 : 32;
+        pk.pad[0] = pk.pad[1] = pk.pad[2] = 0;
+        packs.push_back(pk);
+      }
+      Op o = {TAMD_OP_SMALL, p0, (int)packs.size() - p0, l};
+      o.nojoin = true;
+      o.tile = cls == 0 ? 16 : 32;
+      D->ops.push_back(o);
     }
     if ((int)small.size() > st) {
       Op o = {TAMD_OP_SMALL, st, (int)small.size() - st, l};
-      o.nojoin = true;  // own blocks only: not the parents' blocks zeroed beside it
+      o.nojoin = true;
       D->ops.push_back(o);
     }
     // Two-level blocking: outer panels of W columns, each factored in 64-column
@@ -882,6 +915,13 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     }
   }
   D->ncol = (int)ncol;
+  // GEMM launches of 64-tiles with K <= 64 use the one-round-trip tile
+  for (Op& op : D->ops)
+    if (op.kind == TAMD_OP_GEMM && op.tile == 64) {
+      bool sk = true;
+      for (int q = op.start; q < op.start + op.count && sk; q++) sk = gemm[q].K <= 64;
+      op.shortk = sk;
+    }
   // algorithmic work per op: tile flops (lower half of diagonal tiles), and
   // extend-add / assembly bytes (read child + read-modify-write parent)
   D->op_flops.assign(D->ops.size(), 0.0);
@@ -914,6 +954,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
   D->struct_bytes = sizeof(FrontDesc) * fr.size() + sizeof(int) * (rows.size() + rel.size())
                     + sizeof(long long) * adest.size()
                     + sizeof(int) * (small.size() + colptr.size() + 1)
+                    + sizeof(SmallPack) * packs.size()
                     + sizeof(PanelTask) * (potrf.size() + svt.size())
                     + sizeof(GemmTask) * (trsm.size() + gemm.size()) + sizeof(EATask) * ea.size()
                     + sizeof(ZeroTask) * zero.size()
@@ -921,7 +962,8 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
   if (dry) return 0;
   if (upload(&D->d_fronts, fr, D->stream) || upload(&D->d_rows, rows, D->stream)
       || upload(&D->d_rel, rel, D->stream) || upload(&D->d_adest, adest, D->stream)
-      || upload(&D->d_small, small, D->stream) || upload(&D->d_potrf, potrf, D->stream)
+      || upload(&D->d_small, small, D->stream) || upload(&D->d_packs, packs, D->stream)
+      || upload(&D->d_potrf, potrf, D->stream)
       || upload(&D->d_trsm, trsm, D->stream) || upload(&D->d_gemm, gemm, D->stream)
       || upload(&D->d_ea, ea, D->stream) || upload(&D->d_sv_tasks, svt, D->stream)
       || upload(&D->d_zero, zero, D->stream)
@@ -1168,7 +1210,13 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
              (const int*)D->d_colptr, (const long long*)D->d_adest, (const double*)D->d_avals, L);
         break;
       case TAMD_OP_SMALL:
-        if (g_potrf_wave)
+        if (op.tile == 16)
+          EMIT(lane, join, k_front_slot16<T>, dim3(op.count), dim3(64), (const FrontDesc*)D->d_fronts,
+               (const SmallPack*)(D->d_packs + op.start), L, U, Iv, D->d_info, g_fp64_fallback);
+        else if (op.tile == 32)
+          EMIT(lane, join, k_front_slot32<T>, dim3(op.count), dim3(64), (const FrontDesc*)D->d_fronts,
+               (const SmallPack*)(D->d_packs + op.start), L, U, Iv, D->d_info, g_fp64_fallback);
+        else if (g_potrf_wave)
           EMIT(lane, join, k_front_wave<T>, dim3(op.count), dim3(64), (const FrontDesc*)D->d_fronts,
                (const int*)(D->d_small + op.start), L, U, Iv, D->d_info, g_fp64_fallback);
         else
@@ -1194,6 +1242,9 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
         else if (op.tile == 128)
           EMIT(lane, join, k_gemm_tile128<T>, dim3(op.count), dim3(256),
                (const GemmTask*)(D->d_gemm + op.start), (const T*)L, L, U);
+        else if (op.shortk && lane == 0)
+          EMIT(lane, join, k_gemm_short<T>, dim3(op.count), dim3(256),
+               (const GemmTask*)(D->d_gemm + op.start), (const T*)L, L, U, (const T*)Iv);
         else {
           // The lookahead's trailing update reserves LDS it does not use, so
           // that at most SIDE_WGS<T> of its workgroups share a CU: the panel

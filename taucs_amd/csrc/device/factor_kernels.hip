@@ -305,6 +305,11 @@ __device__ __forceinline__ void wave_chol_inv(T (&r)[64], int lane, unsigned lon
         for (int q = 0; q < 16; q++)
           if (j0 + q < 64) r[j0 + q] -= lik * v[q];
       }
+      // the updated entries, pinned here: left free, the compiler defers each
+      // entry's updates to its own pivot step and keeps every broadcast value
+      // live until then (several hundred registers)
+#pragma clang loop unroll(full)
+      for (int j = k + 1; j < 64; j++) asm volatile("" : "+v"(r[j]));
       __builtin_amdgcn_wave_barrier();
     }
   }
@@ -340,6 +345,7 @@ __device__ __forceinline__ void wave_inverse(T (&r)[64], int lane, T* Ls, const 
         if (t + 1 < i) a1 -= Lb[i * PW_LD + t + 1] * x[t + 1];
       }
       x[i] = (a0 + a1) * rdiag[16 * b + i];
+      asm volatile("" : "+v"(x[i]));  // row i done here (see wave_chol_inv)
     }
     // X(16b+i, 16b+c16) for i >= c16, at the transposed (upper) position
 #pragma clang loop unroll(full)
@@ -583,120 +589,10 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
   if (lane == 0 && bad) atomicOr(info, 1);
 }
 
-// Small fronts packed into one wavefront: fronts with m <= 16 four to a wave,
-// m <= 32 two to a wave, at lane offsets 0/16/32/48 (0/32).  The wave factors
-// the block-diagonal matrix of its fronts: pivot k is eliminated only in the
-// first s columns of its slot's front, and the couplings between slots are
-// zero, so every rank-1 update of another slot's pivot leaves a front's
-// entries exactly unchanged, and the block inverse of the 16 x 16 blocks
-// treats each slot as it treats a front at lane 0.  A front's arithmetic is
-// therefore the same in every slot (results are bitwise those of one front
-// per wave, width 64).  Per front, as k_front_wave: L in the panel columns,
-// the Schur complement in the update block, the inverse of the leading s x s
-// block (its s x s part: the solve reads nothing beyond it).
 template <typename T>
-__global__ __launch_bounds__(64) void k_front_pack(const FrontDesc* __restrict__ fr,
-                                                   const SmallPack* __restrict__ packs,
-                                                   T* __restrict__ L, T* __restrict__ U,
-                                                   T* __restrict__ Inv, int* __restrict__ info,
-                                                   int fallback)
+__device__ __forceinline__ void guw_store(T* gu, int li, int s, int c, int u, T v)
 {
-  __shared__ __align__(16) T col[64];
-  __shared__ __align__(16) T Ls[64 * PW_LD];
-  __shared__ __align__(16) T rdiag[64];
-  __shared__ __align__(16) T Ts[16 * 17];
-  __shared__ __align__(16) T dg[64];
-  const SmallPack* pk = packs + blockIdx.x;
-  const int lane = threadIdx.x, W = pk->width;
-  const int slot = lane / W, base = slot * W, li = lane - base;
-  const int fi = pk->f[slot];
-  int s = 0, m = 0;
-  long long offL = 0, offU = 0, offI = 0;
-  if (fi >= 0) {
-    const FrontDesc d = fr[fi];
-    s = d.s;
-    m = d.m;
-    offL = d.offL;
-    offU = d.offU;
-    offI = d.offI;
-  }
-  const int u = m - s;
-  unsigned long long piv = 0;  // the first s columns of every slot's front
-  for (int q = 0; q * W < 64; q++) {
-    const int f = pk->f[q];
-    if (f >= 0) {
-      const int sq = fr[f].s;
-      piv |= (sq >= 64 ? ~0ULL : (1ULL << sq) - 1) << (q * W);
-    }
-  }
-  T* const gp = L + offL;
-  T* const gu = U + offU;
-  // row `lane` of the wave's block-diagonal matrix, staged through LDS (each
-  // lane's columns are those of its own slot, at run-time offsets)
-  T* row = Ls + lane * PW_LD;
-#pragma unroll
-  for (int c = 0; c < 64; c++) row[c] = c == lane ? (T)1 : (T)0;
-  if (fi >= 0 && li < m) {
-    for (int cl = 0; cl <= li; cl++) {
-      if (cl < s) row[base + cl] = gp[li + (long long)cl * m];
-      else row[base + cl] = gu[(li - s) + (long long)(cl - s) * u];
-    }
-  }
-  wave_lds_fence();
-  T r[64];
-#pragma unroll
-  for (int c = 0; c < 64; c++) r[c] = row[c];
-  if constexpr (sizeof(T) == 4) {
-    dg[lane] = row[lane];
-    wave_lds_fence();
-  }
-  bool bad = false, grow = false;
-  wave_chol_inv<T>(r, lane, piv, col, Ls, rdiag, dg, bad, grow);
-  if constexpr (sizeof(T) == 4) {
-    if (fallback && (bad || grow)) {  // wave-uniform: every front of the wave in FP64
-      double* P = reinterpret_cast<double*>(Ls);
-      for (int c = 0; c <= lane; c++) P[ptri(lane, c)] = c == lane ? 1.0 : 0.0;
-      if (fi >= 0 && li < m) {
-        for (int cl = 0; cl <= li; cl++) {
-          const int c = base + cl;
-          if (cl < s) P[ptri(lane, c)] = (double)gp[li + (long long)cl * m];
-          else P[ptri(lane, c)] = (double)gu[(li - s) + (long long)(cl - s) * u];
-        }
-      }
-      fp64_refactor<T>(r, lane, piv, Ls, rdiag, bad);
-      if (lane == 0) atomicAdd(info + 1, 1);
-    }
-  }
-  // the factored entries back through LDS: L in the panel columns, the Schur
-  // complement in the update block
-  wave_lds_fence();
-#pragma unroll
-  for (int c = 0; c < 64; c++) row[c] = r[c];
-  wave_lds_fence();
-  if (fi >= 0 && li < m) {
-    for (int cl = 0; cl <= li; cl++) {
-      if (cl < s) gp[li + (long long)cl * m] = row[base + cl];
-      else gu[(li - s) + (long long)(cl - s) * u] = row[base + cl];
-    }
-  }
-  // the inverse of each front's leading s x s block: its other rows become the
-  // identity (they hold the Schur complement, not L)
-  if (li >= s) {
-#pragma unroll
-    for (int c = 0; c < 64; c++) r[c] = c == lane ? (T)1 : (T)0;
-    rdiag[lane] = (T)1;
-  }
-  wave_lds_fence();
-  wave_inverse<T>(r, lane, Ls, rdiag, Ts);
-  // lane holds column `lane` of the inverse: X(base + i, lane) = r[base + i]
-#pragma unroll
-  for (int c = 0; c < 64; c++) row[c] = r[c];
-  wave_lds_fence();
-  if (fi >= 0 && li < s) {
-    T* iv = Inv + offI;
-    for (int i = 0; i < s; i++) iv[i * 64 + li] = row[base + i];
-  }
-  if (lane == 0 && bad) atomicOr(info, 1);
+  gu[(li - s) + (long long)(c - s) * u] = v;
 }
 
 // Fronts with m <= 16, four to a wavefront, lane = row within its slot of 16
@@ -735,16 +631,21 @@ __global__ __launch_bounds__(64) void k_front_slot16(const FrontDesc* __restrict
   const int u = m - s;
   T* const gp = L + offL;
   T* const gu = U + offU;
+  // the lane's row of its front, staged through LDS (one load per entry at
+  // run-time column offsets; the registers take it at fixed indices)
+  T* row = Ls + lane * 17;
+#pragma unroll
+  for (int c = 0; c < 16; c++) row[c] = li == c ? (T)1 : (T)0;
+  if (fi >= 0 && li < m) {
+    for (int c = 0; c <= li; c++) {
+      if (c < s) row[c] = gp[li + (long long)c * m];
+      else row[c] = gu[(li - s) + (long long)(c - s) * u];
+    }
+  }
+  wave_lds_fence();
   T r[16];
 #pragma unroll
-  for (int c = 0; c < 16; c++) {
-    T v = li == c ? (T)1 : (T)0;
-    if (fi >= 0 && li < m) {
-      if (c < s) v = gp[li + (long long)c * m];
-      else if (c < m && li >= s) v = gu[(li - s) + (long long)(c - s) * u];
-    }
-    r[c] = v;
-  }
+  for (int c = 0; c < 16; c++) r[c] = row[c];
   if constexpr (sizeof(T) == 4) {
     T v = (T)1;
 #pragma unroll
@@ -777,6 +678,8 @@ __global__ __launch_bounds__(64) void k_front_slot16(const FrontDesc* __restrict
     wave_lds_fence();
 #pragma clang loop unroll(full)
     for (int j = k + 1; j < 16; j++) r[j] -= lik * col[base + j];
+#pragma clang loop unroll(full)
+    for (int j = k + 1; j < 16; j++) asm volatile("" : "+v"(r[j]));  // as in wave_chol_inv
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -806,11 +709,16 @@ __global__ __launch_bounds__(64) void k_front_slot16(const FrontDesc* __restrict
       if (lane == 0) atomicAdd(info + 1, 1);
     }
   }
+  // the factored row back through LDS: L in the panel columns, the Schur
+  // complement in the update block
+  wave_lds_fence();
 #pragma unroll
-  for (int c = 0; c < 16; c++) {
-    if (fi >= 0 && li < m && li >= c) {
-      if (c < s) gp[li + (long long)c * m] = r[c];
-      else if (c < m) gu[(li - s) + (long long)(c - s) * u] = r[c];
+  for (int c = 0; c < 16; c++) row[c] = r[c];
+  wave_lds_fence();
+  if (fi >= 0 && li < m) {
+    for (int c = 0; c <= li; c++) {
+      if (c < s) gp[li + (long long)c * m] = row[c];
+      else guw_store(gu, li, s, c, u, row[c]);
     }
   }
   // inverse of each front's leading s x s block (its other rows: identity)
@@ -827,6 +735,9 @@ __global__ __launch_bounds__(64) void k_front_slot16(const FrontDesc* __restrict
   T x[16];
 #pragma clang loop unroll(full)
   for (int i = 0; i < 16; i++) {
+    // a compiler-only fence per row: the row's entries are loaded where they
+    // are used, so the 120 entries never occupy registers at once
+    asm volatile("" ::: "memory");
     T a0 = (i == li) ? (T)1 : (T)0, a1 = (T)0;
 #pragma clang loop unroll(full)
     for (int t = 0; t < i; t += 2) {
@@ -834,13 +745,220 @@ __global__ __launch_bounds__(64) void k_front_slot16(const FrontDesc* __restrict
       if (t + 1 < i) a1 -= Lb[i * 17 + t + 1] * x[t + 1];
     }
     x[i] = (a0 + a1) * rdiag[base + i];
-    __builtin_amdgcn_sched_barrier(0);  // one row's loads at a time (registers)
+    asm volatile("" : "+v"(x[i]));
   }
   if (fi >= 0 && li < s) {
     T* iv = Inv + offI;
 #pragma unroll
     for (int i = 0; i < 16; i++)
       if (i < s) iv[i * 64 + li] = i >= li ? x[i] : (T)0;
+  }
+  if (bad) atomicOr(info, 1);
+}
+
+// Fronts with 16 < m <= 32, two to a wavefront (lanes 0-31 and 32-63), each
+// lane holding only its slot's 32 columns.  The Cholesky steps are those of
+// k_front_slot16; the inverse follows wave_inverse on the wave's block-
+// diagonal matrix: the four 16 x 16 diagonal blocks (two per front) by
+// substitution, then only the block pairs (1,0) and (3,2), each a front's own
+// off-diagonal block, by the same MFMA products.  A front in either slot goes
+// through exactly the operations k_front_wave applies to it at lane 0, so the
+// results are bitwise those of one front per wave.
+template <typename T>
+__global__ __launch_bounds__(64) void k_front_slot32(const FrontDesc* __restrict__ fr,
+                                                     const SmallPack* __restrict__ packs,
+                                                     T* __restrict__ L, T* __restrict__ U,
+                                                     T* __restrict__ Inv, int* __restrict__ info,
+                                                     int fallback)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int NSB = 64 * PW_LD * sizeof(T) > 2080 * 8 ? 64 * PW_LD * sizeof(T) : 2080 * 8;
+  __shared__ __align__(16) unsigned char sbuf[NSB];
+  __shared__ __align__(16) T col[64];
+  __shared__ __align__(16) T rdiag[64];
+  __shared__ __align__(16) T dg[64];
+  __shared__ __align__(16) T Ts[16 * 17];
+  T* Ls = reinterpret_cast<T*>(sbuf);  // 64 x PW_LD, columns indexed within the wave
+  const SmallPack* pk = packs + blockIdx.x;
+  const int lane = threadIdx.x, slot = lane >> 5, base = slot << 5, li = lane & 31;
+  const int fi = pk->f[slot];
+  int s = 0, m = 0;
+  long long offL = 0, offU = 0, offI = 0;
+  if (fi >= 0) {
+    const FrontDesc d = fr[fi];
+    s = d.s;
+    m = d.m;
+    offL = d.offL;
+    offU = d.offU;
+    offI = d.offI;
+  }
+  const int u = m - s;
+  T* const gp = L + offL;
+  T* const gu = U + offU;
+  // the lane's row of its front, staged through LDS (one load per entry at
+  // run-time column offsets; the registers take it at fixed indices)
+  T* row = Ls + lane * PW_LD + base;
+#pragma unroll
+  for (int c = 0; c < 32; c++) row[c] = li == c ? (T)1 : (T)0;
+  if (fi >= 0 && li < m) {
+    for (int c = 0; c <= li; c++) {
+      if (c < s) row[c] = gp[li + (long long)c * m];
+      else row[c] = gu[(li - s) + (long long)(c - s) * u];
+    }
+  }
+  wave_lds_fence();
+  T r[32];
+#pragma unroll
+  for (int c = 0; c < 32; c++) r[c] = row[c];
+  if constexpr (sizeof(T) == 4) {
+    T v = (T)1;
+#pragma unroll
+    for (int c = 0; c < 32; c++)
+      if (c == li) v = r[c];
+    dg[lane] = v;
+    wave_lds_fence();
+  }
+  bool bad = false, grow = false;
+#pragma clang loop unroll(full)
+  for (int k = 0; k < 32; k++) {
+    const T p0 = rlane(r[k], k), p1 = rlane(r[k], 32 + k);
+    T p = slot == 0 ? p0 : p1;
+    const bool act = k < s;
+    if (act) {
+      if constexpr (sizeof(T) == 4) {
+        if (p < (T)GROW_TAU * dg[base + k]) grow = true;
+      }
+      if (!(p > (T)0)) { bad = true; p = (T)1; }
+    }
+    T dk, rdk;
+    sqrt_pair(p, dk, rdk);
+    const T lik = act ? r[k] * rdk : (T)0;
+    if (act) {
+      r[k] = li == k ? dk : lik;
+      if (li == k) rdiag[lane] = rdk;
+    }
+    col[lane] = r[k];
+    wave_lds_fence();
+#pragma clang loop unroll(full)
+    for (int j0 = k + 1; j0 < 32; j0 += 16) {
+      T v[16];
+#pragma clang loop unroll(full)
+      for (int q = 0; q < 16; q++)
+        if (j0 + q < 32) v[q] = col[base + j0 + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma clang loop unroll(full)
+      for (int q = 0; q < 16; q++)
+        if (j0 + q < 32) r[j0 + q] -= lik * v[q];
+    }
+#pragma clang loop unroll(full)
+    for (int j = k + 1; j < 32; j++) asm volatile("" : "+v"(r[j]));  // as in wave_chol_inv
+    __builtin_amdgcn_wave_barrier();
+  }
+  if constexpr (sizeof(T) == 4) {
+    // a flagged slot anywhere in the wave: every front of the wave in FP64
+    if (fallback && __any_sync(~0ULL, bad || grow)) {
+      double* P = reinterpret_cast<double*>(sbuf);
+      unsigned long long piv = 0;
+      for (int q = 0; q < 2; q++) {
+        const int f = pk->f[q];
+        if (f >= 0) piv |= ((1ULL << fr[f].s) - 1) << (32 * q);
+      }
+      for (int c = 0; c <= lane; c++) P[ptri(lane, c)] = c == lane ? 1.0 : 0.0;
+      if (fi >= 0 && li < m) {
+        for (int cl = 0; cl <= li; cl++) {
+          if (cl < s) P[ptri(lane, base + cl)] = (double)gp[li + (long long)cl * m];
+          else P[ptri(lane, base + cl)] = (double)gu[(li - s) + (long long)(cl - s) * u];
+        }
+      }
+      bool bad64 = false;
+      wave_chol_packed_f64(P, lane, piv, bad64);
+#pragma unroll
+      for (int c = 0; c < 32; c++) r[c] = c <= li ? (T)P[ptri(lane, base + c)] : (T)0;
+      rdiag[lane] = li < s ? (T)(1.0 / P[ptri(lane, lane)]) : (T)1;
+      wave_lds_fence();
+      bad = bad64;
+      if (lane == 0) atomicAdd(info + 1, 1);
+    }
+  }
+  // the factored row back through LDS: L in the panel columns, the Schur
+  // complement in the update block
+  wave_lds_fence();
+#pragma unroll
+  for (int c = 0; c < 32; c++) row[c] = r[c];
+  wave_lds_fence();
+  if (fi >= 0 && li < m) {
+    for (int c = 0; c <= li; c++) {
+      if (c < s) gp[li + (long long)c * m] = row[c];
+      else guw_store(gu, li, s, c, u, row[c]);
+    }
+  }
+  // inverse of each front's leading s x s block (its other rows: identity)
+  if (li >= s) {
+#pragma unroll
+    for (int c = 0; c < 32; c++) r[c] = c == li ? (T)1 : (T)0;
+    rdiag[lane] = (T)1;
+  }
+  wave_lds_fence();
+#pragma unroll
+  for (int c = 0; c < 32; c++) Ls[lane * PW_LD + base + c] = r[c];
+  wave_lds_fence();
+  const int b = lane >> 4, c16 = lane & 15, kq = lane >> 4;
+  {
+    T x[16];
+    const T* Lb = Ls + (16 * b) * PW_LD + 16 * b;
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 16; i++) {
+      asm volatile("" ::: "memory");  // one row's entries in registers at a time
+      T a0 = (i == c16) ? (T)1 : (T)0, a1 = (T)0;
+#pragma clang loop unroll(full)
+      for (int t = 0; t < i; t += 2) {
+        a0 -= Lb[i * PW_LD + t] * x[t];
+        if (t + 1 < i) a1 -= Lb[i * PW_LD + t + 1] * x[t + 1];
+      }
+      x[i] = (a0 + a1) * rdiag[16 * b + i];
+      asm volatile("" : "+v"(x[i]));
+    }
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 16; i++)
+      if (i >= c16) Ls[(16 * b + c16) * PW_LD + 16 * b + i] = x[i];
+  }
+  wave_lds_fence();
+  // each front's off-diagonal block: X_10 = -X_11 (L_10 X_00), pairs (1,0), (3,2)
+#pragma clang loop unroll(full)
+  for (int q = 0; q < 2; q++) {
+    const int i = 2 * q + 1, j = 2 * q;
+    acc_t acc;
+#pragma clang loop unroll(full)
+    for (int e = 0; e < 4; e++) acc[e] = (T)0;
+#pragma clang loop unroll(full)
+    for (int st = 0; st < 4; st++) {
+      const int kr = 4 * st + kq;
+      const T a = Ls[(16 * i + c16) * PW_LD + 16 * j + kr];
+      const T xv = kr >= c16 ? Ls[(16 * j + c16) * PW_LD + 16 * j + kr] : (T)0;
+      acc = Mfma<T>::step(a, xv, acc);
+    }
+#pragma clang loop unroll(full)
+    for (int e = 0; e < 4; e++) Ts[Mfma<T>::drow(lane, e) * 17 + c16] = acc[e];
+    wave_lds_fence();
+    acc_t xr;
+#pragma clang loop unroll(full)
+    for (int e = 0; e < 4; e++) xr[e] = (T)0;
+#pragma clang loop unroll(full)
+    for (int st = 0; st < 4; st++) {
+      const int kr = 4 * st + kq;
+      const T a = (c16 >= kr) ? Ls[(16 * i + kr) * PW_LD + 16 * i + c16] : (T)0;
+      xr = Mfma<T>::step(a, Ts[kr * 17 + c16], xr);
+    }
+#pragma clang loop unroll(full)
+    for (int e = 0; e < 4; e++)
+      Ls[(16 * j + c16) * PW_LD + 16 * i + Mfma<T>::drow(lane, e)] = -xr[e];
+    wave_lds_fence();
+  }
+  // lane holds column `lane` of its front's inverse: X(base + i, lane)
+  if (fi >= 0 && li < s) {
+    T* iv = Inv + offI;
+    for (int i = 0; i < s; i++)
+      iv[i * 64 + li] = i >= li ? Ls[lane * PW_LD + base + i] : (T)0;
   }
   if (bad) atomicOr(info, 1);
 }
@@ -956,6 +1074,95 @@ __device__ __forceinline__ void gemm_tile(const GemmTask* __restrict__ tasks, co
       }
 }
 
+// The same tile for K <= 64 (the panel chain's TRSM and the panel-internal
+// updates), shaped for latency: the four 16-deep chunks are fetched at once,
+// so the tile pays one global round trip instead of four, and are staged
+// through LDS two chunks at a time.  The chunks, their LDS layout and the
+// MFMA order are those of gemm_tile: the results are bitwise equal.
+template <typename T>
+__device__ __forceinline__ void gemm_tile_short(const GemmTask* __restrict__ tasks, const T* Lsrc,
+                                                T* L, T* U, const T* __restrict__ Inv)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ __align__(16) T As[2][64][KP];
+  __shared__ __align__(16) T Bs[2][64][KP];
+  const GemmTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const T* A = Lsrc + t.a_off;
+  const bool binv = (t.flags & 4) != 0;
+  const T* B = binv ? Inv + t.b_off : Lsrc + t.b_off;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
+
+  const int li = tid & 63, lk = tid >> 6;  // loader: row li, k = 16 ch + lk + 4q
+  T ra[4][TK / 4], rb[4][TK / 4];
+#pragma unroll
+  for (int ch = 0; ch < 4; ch++)
+#pragma unroll
+    for (int q = 0; q < TK / 4; q++) {
+      const int k = ch * TK + lk + 4 * q;
+      const bool kin = k < t.K;
+      ra[ch][q] = (kin && li < t.M) ? A[li + (long long)k * t.ld] : (T)0;
+      rb[ch][q] = (kin && li < t.N) ? (binv ? B[li * 64 + k] : B[li + (long long)k * t.ld]) : (T)0;
+    }
+  const int c16 = lane & 15, g = lane >> 4;
+  const int nch = (t.K + TK - 1) / TK;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    if (2 * h >= nch) break;
+    if (h) __syncthreads();  // the first half's fragments are read
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int q = 0; q < TK / 4; q++) {
+        As[c][li][lk + 4 * q] = ra[2 * h + c][q];
+        Bs[c][li][lk + 4 * q] = rb[2 * h + c][q];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      if (2 * h + c >= nch) break;
+      T fa0[TK / 4], fa1[TK / 4], fb0[TK / 4], fb1[TK / 4];
+#pragma unroll
+      for (int s2 = 0; s2 < TK / 4; s2++) {
+        fa0[s2] = As[c][wr + c16][g * (TK / 4) + s2];
+        fa1[s2] = As[c][wr + 16 + c16][g * (TK / 4) + s2];
+        fb0[s2] = Bs[c][wc + c16][g * (TK / 4) + s2];
+        fb1[s2] = Bs[c][wc + 16 + c16][g * (TK / 4) + s2];
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < TK / 4; s2++) {
+        acc[0][0] = Mfma<T>::step(fb0[s2], fa0[s2], acc[0][0]);
+        acc[0][1] = Mfma<T>::step(fb1[s2], fa0[s2], acc[0][1]);
+        acc[1][0] = Mfma<T>::step(fb0[s2], fa1[s2], acc[1][0]);
+        acc[1][1] = Mfma<T>::step(fb1[s2], fa1[s2], acc[1][1]);
+      }
+    }
+  }
+
+  T* C = ((t.flags & 1) ? U : L) + t.c_off;
+  const bool diag = (t.flags & 2) != 0;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int i = wr + a * 16 + (lane & 15);
+        int j = wc + b * 16 + Mfma<T>::drow(lane, r);
+        if (i < t.M && j < t.N && (!diag || i >= j)) {
+          if (binv) C[i + (long long)j * t.ldc] = acc[a][b][r];
+          else C[i + (long long)j * t.ldc] -= acc[a][b][r];
+        }
+      }
+}
+
 // Trailing update / SYRK tiles (C -= A B^T) and the panel solves (C = A
 // inv(L_kk)^T, flag 4) share the tile code; two symbols keep their launches
 // apart in profiles.
@@ -972,7 +1179,16 @@ __global__ __launch_bounds__(256) void k_trsm_tile(const GemmTask* __restrict__ 
                                                    const T* __restrict__ Inv)
 {
   __builtin_amdgcn_s_setprio(2);
-  gemm_tile<T, 1>(tasks, Lsrc, L, U, Inv);
+  gemm_tile_short<T>(tasks, Lsrc, L, U, Inv);
+}
+// Panel-internal updates (K <= 64), on the panel chain's critical path.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_short(const GemmTask* __restrict__ tasks,
+                                                    const T* Lsrc, T* L, T* U,
+                                                    const T* __restrict__ Inv)
+{
+  __builtin_amdgcn_s_setprio(2);
+  gemm_tile_short<T>(tasks, Lsrc, L, U, Inv);
 }
 
 // 128 x 128 tile for the large trailing updates and SYRKs: 4 waves (2 x 2),
@@ -1036,98 +1252,6 @@ __global__ __launch_bounds__(256) void k_gemm_tile128(const GemmTask* __restrict
         for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::step(bf[b], af[a], acc[a][b]);
     }
     __syncthreads();
-  }
-
-  T* C = ((t.flags & 1) ? U : L) + t.c_off;
-  const bool diag = (t.flags & 2) != 0;
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int i = wr + a * 16 + (lane & 15);
-        const int j = wc + b * 16 + Mfma<T>::drow(lane, r);
-        if (i < t.M && j < t.N && (!diag || i >= j)) C[i + (long long)j * t.ldc] -= acc[a][b][r];
-      }
-}
-
-// 128 x 128 tile without LDS: each wave reads its own MFMA operands straight
-// from global memory (lane (c16, g) of k-step s: row c16 of each 16-row block
-// at k = 4s + g, the layout the 16x16x4 MFMA takes), two k-steps ahead in
-// registers, so there is no barrier and no LDS traffic; the two waves that
-// share a row (column) block find its lines in L1.  Rows beyond M and columns
-// beyond N are clamped to the last valid one (their results are not stored);
-// a K tail is zero-filled.  The k order of every sum is that of
-// k_gemm_tile128, so the two kernels give bitwise equal results.
-template <typename T>
-__global__ __launch_bounds__(256, 2) void k_gemm_tile128d(const GemmTask* __restrict__ tasks,
-                                                         const T* Lsrc, T* L, T* U)
-{
-  typedef typename Mfma<T>::acc_t acc_t;
-  const GemmTask t = tasks[blockIdx.x];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-  const int c16 = lane & 15, g = lane >> 4;
-  const long long ld = t.ld;
-  const T* pa[4];
-  const T* pb[4];
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    const int ia = min(wr + 16 * a + c16, t.M - 1), jb = min(wc + 16 * a + c16, t.N - 1);
-    pa[a] = Lsrc + t.a_off + ia + (long long)g * ld;
-    pb[a] = Lsrc + t.b_off + jb + (long long)g * ld;
-  }
-  acc_t acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
-  const int K = t.K, nfull = K >> 2;
-  const long long step = 4 * ld;
-  T fa[3][4], fb[3][4];  // a ring of three k-steps of operands
-  auto load = [&](int slot, int ks) {
-    if (ks < nfull) {
-      const long long o = (long long)ks * step;
-#pragma unroll
-      for (int a = 0; a < 4; a++) {
-        fa[slot][a] = pa[a][o];
-        fb[slot][a] = pb[a][o];
-      }
-    }
-  };
-  auto mma = [&](int slot, int ks) {
-    if (ks < nfull) {
-#pragma unroll
-      for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::step(fb[slot][b], fa[slot][a], acc[a][b]);
-    }
-  };
-  load(0, 0);
-  load(1, 1);
-  for (int ks = 0; ks < nfull; ks += 3) {
-    load(2, ks + 2);
-    mma(0, ks);
-    load(0, ks + 3);
-    mma(1, ks + 1);
-    load(1, ks + 4);
-    mma(2, ks + 2);
-  }
-  if (K & 3) {  // tail: k = 4 nfull + g exists only for g < K mod 4
-    const bool kin = g < (K & 3);
-    const long long o = (long long)nfull * step;
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-      fa[0][a] = kin ? pa[a][o] : (T)0;
-      fb[0][a] = kin ? pb[a][o] : (T)0;
-    }
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::step(fb[0][b], fa[0][a], acc[a][b]);
   }
 
   T* C = ((t.flags & 1) ? U : L) + t.c_off;
@@ -1296,12 +1420,12 @@ __global__ __launch_bounds__(256) void k_extend_add_seq(const FrontDesc* __restr
   template __global__ void k_potrf_diag<T>(const FrontDesc*, const PanelTask*, T*, T*, int*); \
   template __global__ void k_potrf_wave<T>(const FrontDesc*, const PanelTask*, T*, T*, int*, int); \
   template __global__ void k_front_wave<T>(const FrontDesc*, const int*, T*, T*, T*, int*, int); \
-  template __global__ void k_front_pack<T>(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int); \
   template __global__ void k_front_slot16<T>(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int); \
+  template __global__ void k_front_slot32<T>(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int); \
   template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_trsm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_gemm_tile128<T>(const GemmTask*, const T*, T*, T*);          \
-  template __global__ void k_gemm_tile128d<T>(const GemmTask*, const T*, T*, T*);         \
+  template __global__ void k_gemm_short<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*); \
   template __global__ void k_extend_add_seq<T>(const FrontDesc*, const EATask*, const EATask*,   \
                                                const int*, T*, T*);

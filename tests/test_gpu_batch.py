@@ -193,3 +193,27 @@ def test_numeric_from_device_values_equals_host_values():
     dVb = T.DeviceArray(np.concatenate([m.arrays()[3] for m in mats]))
     assert B1.numeric_device(dVb.ptr) == 0
     assert np.array_equal(B1.solve(bb), xb)
+
+
+@pytest.mark.parametrize("prec", [T.FP64, T.FP32])
+def test_front_packing_is_bitwise_identical(prec):
+    """Fronts of at most 16 (32) rows factored four (two) to a wavefront give
+    the bits of one front per wavefront: factor digest and solution."""
+    for name, A in (("mesh2d 60", T.mesh2d(60)), ("mesh3d 16", T.mesh3d(16)),
+                    ("shifted mesh2d 90", T.mesh2d_shifted(90, seed=3))):
+        print(f"case: front packing, {name}, precision {prec}", flush=True)
+        perm, inv = T.order(A, "genmmd")
+        PA = T.permute(A, perm, inv)
+        b = T.random_uniform(PA.n, 17)
+        res = []
+        try:
+            for on in (1, 0):
+                T.lib.taucs_amd_set_front_packing(on)
+                F = T.factor_symbolic(PA)
+                assert F.numeric(PA, prec) == 0
+                res.append((F.digest(), F.solve(b)))
+                F.free()
+        finally:
+            T.lib.taucs_amd_set_front_packing(1)
+        assert res[0][0] == res[1][0]
+        assert np.array_equal(res[0][1], res[1][1])
