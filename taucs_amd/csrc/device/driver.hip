@@ -23,6 +23,7 @@ template <typename T> __global__ void k_assemble(int, const int*, const long lon
 template <typename T> __global__ void k_front_small(const FrontDesc*, const int*, T*, T*, T*, int*);
 template <typename T> __global__ void k_potrf_diag(const FrontDesc*, const PanelTask*, T*, T*, int*);
 template <typename T> __global__ void k_potrf_wave(const FrontDesc*, const PanelTask*, T*, T*, int*, int);
+template <typename T> __global__ void k_potrf_mw(const FrontDesc*, const PanelTask*, T*, T*, int*, int);
 template <typename T> __global__ void k_front_wave(const FrontDesc*, const int*, T*, T*, T*, int*, int);
 template <typename T> __global__ void k_front_slot16(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int);
 template <typename T> __global__ void k_front_slot32(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int);
@@ -167,10 +168,11 @@ static int g_tile_big = 64;
 extern "C" void taucs_amd_set_gemm_tile(int edge) { g_tile_big = edge == 128 ? 128 : 64; }
 static int g_lookahead = 1;
 extern "C" void taucs_amd_set_lookahead(int on) { g_lookahead = on; }
-// Diagonal-block kernel: 1 = one wave with register broadcasts, 0 = 256
-// threads through LDS (kept for comparison).
-static int g_potrf_wave = 1;
-extern "C" void taucs_amd_set_potrf_kernel(int wave) { g_potrf_wave = wave; }
+// Diagonal-block kernel: 2 = four waves, 16-column panels with MFMA updates
+// (default); 1 = one wave with register broadcasts; 0 = 256 threads through
+// LDS (kept for comparison).  2 and 1 give bitwise equal factors.
+static int g_potrf_wave = 2;
+extern "C" void taucs_amd_set_potrf_kernel(int kind) { g_potrf_wave = kind >= 0 && kind <= 2 ? kind : 2; }
 // Small fronts several to a wavefront: m <= 16 four per wave (k_front_slot16),
 // m <= 32 two per wave (k_front_slot32); results are bitwise those of one
 // front per wave.  Applies to schedules built afterwards (default on).
@@ -1224,7 +1226,10 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
                (const int*)(D->d_small + op.start), L, U, Iv, D->d_info);
         break;
       case TAMD_OP_POTRF:
-        if (g_potrf_wave)
+        if (g_potrf_wave == 2)
+          EMIT(lane, join, k_potrf_mw<T>, dim3(op.count), dim3(256), (const FrontDesc*)D->d_fronts,
+               (const PanelTask*)(D->d_potrf + op.start), L, Iv, D->d_info, g_fp64_fallback);
+        else if (g_potrf_wave)
           EMIT(lane, join, k_potrf_wave<T>, dim3(op.count), dim3(64), (const FrontDesc*)D->d_fronts,
                (const PanelTask*)(D->d_potrf + op.start), L, Iv, D->d_info, g_fp64_fallback);
         else
@@ -1321,7 +1326,7 @@ static int run_numeric(tamd_dev* D, bool bf16upd)
     HIPCHECK(hipEventRecord(e0, st));
   }
   const int key = (int)sizeof(T) * 2 + (bf16upd ? 1 : 0) + (g_lookahead ? 16 : 0)
-                  + (g_potrf_wave ? 32 : 0) + (g_fp64_fallback ? 64 : 0);
+                  + 32 * g_potrf_wave + (g_fp64_fallback ? 128 : 0);
   if (prof || !g_graphs) {
     Emitter E = eager_emitter(D);
     if (launch_numeric<T>(D, E, bf16upd, prof) != 0) return -1;

@@ -495,6 +495,230 @@ __global__ __launch_bounds__(64) void k_potrf_wave(const FrontDesc* __restrict__
   if (lane == 0 && bad) atomicOr(info, 1);
 }
 
+// The diagonal block of a panel step by four waves (256 threads).  The
+// one-wave kernel above is bound by its instruction stream: one wave issues
+// every pivot's chain (broadcast, square root, scaling) and all 2016 rank-1
+// FMAs, so FP32 runs no faster than FP64.  Here the block lives in LDS
+// (column-major, A(i, c) at A[c*PM_LDA + i]) and is factored in 16-column
+// panels: wave 0 factors panel p in registers (lane = row, its 16 panel
+// entries; pivot by lane broadcast and sqrt_pair, the panel's own updates
+// with the multipliers broadcast by readlane), then all four waves apply the
+// panel to the 16 x 16 blocks right of it as rank-16 MFMA updates,
+// C -= L_Qp L_Rp^T.  The inverse X = L^{-1} follows in the same LDS block,
+// transposed into its upper triangle: the four 16 x 16 diagonal blocks by
+// substitution (one wave, lane = (block, column)), then the off-diagonal
+// blocks by distance on MFMA, X_ij = -X_ii sum_k L_ik X_kj, one block per
+// wave.  Loads and stores go through LDS, coalesced.  The MFMA steps apply
+// their products as one fused multiply-add after another in k order, so L
+// and the inverse are bitwise those of k_potrf_wave.
+#define PM_LDA 65
+
+// C(Q, R) -= L(Q, p) L(R, p)^T on 16 x 16 blocks of A, by one wave; on a
+// diagonal block only its lower triangle is written
+template <typename T>
+__device__ __forceinline__ void pm_block_update(T* A, int Q, int R, int p, int lane)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int c16 = lane & 15, g = lane >> 4;
+  acc_t acc;
+#pragma clang loop unroll(full)
+  for (int e = 0; e < 4; e++) acc[e] = A[(16 * R + c16) * PM_LDA + 16 * Q + Mfma<T>::drow(lane, e)];
+#pragma clang loop unroll(full)
+  for (int st = 0; st < 4; st++) {
+    const int k = 16 * p + 4 * st + g;
+    const T a = -A[k * PM_LDA + 16 * Q + c16];
+    const T b = A[k * PM_LDA + 16 * R + c16];
+    acc = Mfma<T>::step(a, b, acc);
+  }
+#pragma clang loop unroll(full)
+  for (int e = 0; e < 4; e++) {
+    const int i = Mfma<T>::drow(lane, e);
+    if (Q != R || i >= c16) A[(16 * R + c16) * PM_LDA + 16 * Q + i] = acc[e];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_mw(const FrontDesc* __restrict__ fr,
+                                                  const PanelTask* __restrict__ tasks,
+                                                  T* __restrict__ L, T* __restrict__ Inv,
+                                                  int* __restrict__ info, int fallback)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  // L in the lower triangle; its inverse X, transposed, in the upper one
+  // (X(r, c) at A[r*PM_LDA + c], r >= c; L's diagonal is kept in rdiag)
+  __shared__ __align__(16) T A[64 * PM_LDA];
+  constexpr int NPK = sizeof(T) == 4 ? 2080 : 1;  // FP32: the FP64 fallback's packed triangle
+  __shared__ __align__(16) double Pk[NPK];
+  __shared__ __align__(16) T rdiag[64];
+  __shared__ __align__(16) T dg[64];
+  __shared__ __align__(16) T Ts[4][16 * 17];
+  __shared__ int flagged;
+  __builtin_amdgcn_s_setprio(3);  // the panel chain is the critical path
+  const PanelTask t = tasks[blockIdx.x];
+  const FrontDesc d = fr[t.front];
+  const long long m = d.m;
+  const int kb = t.kb, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  T* const gp = L + d.offL + t.k0 + (long long)t.k0 * m;
+  // the lower triangle (identity padding beyond kb, zeros above)
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int c = wave + 4 * q;
+    T v = lane == c ? (T)1 : (T)0;
+    if (lane < kb && c < kb && lane >= c) v = gp[lane + c * m];
+    A[c * PM_LDA + lane] = v;
+    if (lane == c) dg[c] = v;
+  }
+  if (tid == 0) flagged = 0;
+  __syncthreads();
+  bool bad = false, grow = false;
+#pragma clang loop unroll(full)
+  for (int p = 0; p < 4; p++) {
+    if (wave == 0) {
+      const int c0 = 16 * p;
+      T r[16];
+#pragma unroll
+      for (int c = 0; c < 16; c++) r[c] = A[(c0 + c) * PM_LDA + lane];
+      T myrd = (T)1;
+#pragma clang loop unroll(full)
+      for (int q = 0; q < 16; q++) {
+        const int k = c0 + q;
+        T pv = rlane(r[q], k);
+        if constexpr (sizeof(T) == 4) {
+          if (pv < (T)GROW_TAU * dg[k]) grow = true;
+        }
+        // a nonpositive pivot is recorded beside the chain; its NaNs or
+        // infinities run on through a factor that is reported as failed
+        bad = bad || !(pv > (T)0);
+        T dk, rdk;
+        sqrt_pair(pv, dk, rdk);
+        const T lik = r[q] * rdk;
+        r[q] = lane == k ? dk : lik;
+        myrd = lane == k ? rdk : myrd;
+        // (no register barrier: 16 columns cannot spill, and the scheduler
+        // can then overlap the next pivot's chain with these updates)
+#pragma clang loop unroll(full)
+        for (int j = q + 1; j < 16; j++) r[j] -= lik * rlane(lik, c0 + j);
+      }
+#pragma unroll
+      for (int c = 0; c < 16; c++) A[(c0 + c) * PM_LDA + lane] = lane >= c0 + c ? r[c] : (T)0;
+      if (lane >= c0 && lane < c0 + 16) rdiag[lane] = myrd;
+    }
+    __syncthreads();
+    if (p < 3) {
+      // the blocks (Q, R), Q >= R > p, round robin over the waves
+      int n = 0;
+#pragma clang loop unroll(full)
+      for (int R = p + 1; R < 4; R++)
+#pragma clang loop unroll(full)
+        for (int Q = R; Q < 4; Q++) {
+          if ((n & 3) == wave) pm_block_update<T>(A, Q, R, p, lane);
+          n++;
+        }
+      __syncthreads();
+    }
+  }
+  if constexpr (sizeof(T) == 4) {
+    // a flagged block (nonpositive pivot or pivot growth) again in FP64 from
+    // its input, by wave 0 in the packed triangle
+    if (wave == 0 && fallback && (bad || grow) && lane == 0) flagged = 1;
+    __syncthreads();
+    if (flagged) {
+      if (wave == 0) {
+        double* P = Pk;
+        for (int c = 0; c <= lane; c++)
+          P[ptri(lane, c)] = (lane < kb && c < kb) ? (double)gp[lane + c * m] : (lane == c ? 1.0 : 0.0);
+        bool bad64 = false;
+        wave_chol_packed_f64(P, lane, ~0ULL, bad64);
+        T rd = (T)(1.0 / P[ptri(lane, lane)]);
+        T v[64];
+#pragma unroll
+        for (int c = 0; c < 64; c++) v[c] = c <= lane ? (T)P[ptri(lane, c)] : (T)0;
+        wave_lds_fence();
+#pragma unroll
+        for (int c = 0; c < 64; c++) A[c * PM_LDA + lane] = v[c];
+        rdiag[lane] = rd;
+        bad = bad64;
+        if (lane == 0) atomicAdd(info + 1, 1);
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0 && bad) atomicOr(info, 1);
+  // L back to the panel
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int c = wave + 4 * q;
+    if (lane < kb && c < kb && lane >= c) gp[lane + c * m] = A[c * PM_LDA + lane];
+  }
+  __syncthreads();  // the diagonal of A is overwritten by X's next
+  // inverse: the diagonal 16 x 16 blocks, lane (b, c16) holds column c16 of X_bb
+  const int c16 = lane & 15, g = lane >> 4;
+  if (wave == 0) {
+    const int b = g;
+    T x[16];
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 16; i++) {
+      asm volatile("" ::: "memory");  // one row's entries in registers at a time
+      T a0 = (i == c16) ? (T)1 : (T)0, a1 = (T)0;
+#pragma clang loop unroll(full)
+      for (int tt = 0; tt < i; tt += 2) {
+        a0 -= A[(16 * b + tt) * PM_LDA + 16 * b + i] * x[tt];
+        if (tt + 1 < i) a1 -= A[(16 * b + tt + 1) * PM_LDA + 16 * b + i] * x[tt + 1];
+      }
+      x[i] = (a0 + a1) * rdiag[16 * b + i];
+      asm volatile("" : "+v"(x[i]));
+    }
+    wave_lds_fence();  // every row of L_bb read before X_bb takes the upper triangle
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+      if (i >= c16) A[(16 * b + i) * PM_LDA + 16 * b + c16] = x[i];
+  }
+  __syncthreads();
+  // off-diagonal blocks by distance: X_ij = -X_ii (sum_{k=j}^{i-1} L_ik X_kj);
+  // X_kj's entries above its diagonal (k = j) and X_ii's are masked: there
+  // the upper triangle holds L
+#pragma clang loop unroll(full)
+  for (int dd = 1; dd < 4; dd++) {
+    const int j = wave, i = wave + dd;
+    if (i < 4) {
+      acc_t acc;
+#pragma clang loop unroll(full)
+      for (int e = 0; e < 4; e++) acc[e] = (T)0;
+      for (int k = j; k < i; k++)
+#pragma clang loop unroll(full)
+        for (int st = 0; st < 4; st++) {
+          const int kr = 4 * st + g;
+          const T a = A[(16 * k + kr) * PM_LDA + 16 * i + c16];
+          const T bv = (k > j || kr >= c16) ? A[(16 * k + kr) * PM_LDA + 16 * j + c16] : (T)0;
+          acc = Mfma<T>::step(a, bv, acc);
+        }
+      T* ts = Ts[wave];
+#pragma clang loop unroll(full)
+      for (int e = 0; e < 4; e++) ts[Mfma<T>::drow(lane, e) * 17 + c16] = acc[e];
+      wave_lds_fence();
+      acc_t xr;
+#pragma clang loop unroll(full)
+      for (int e = 0; e < 4; e++) xr[e] = (T)0;
+#pragma clang loop unroll(full)
+      for (int st = 0; st < 4; st++) {
+        const int kr = 4 * st + g;
+        const T a = c16 >= kr ? A[(16 * i + c16) * PM_LDA + 16 * i + kr] : (T)0;
+        xr = Mfma<T>::step(a, ts[kr * 17 + c16], xr);
+      }
+#pragma clang loop unroll(full)
+      for (int e = 0; e < 4; e++) A[(16 * i + Mfma<T>::drow(lane, e)) * PM_LDA + 16 * j + c16] = -xr[e];
+    }
+    __syncthreads();
+  }
+  // Inv row-major, X(i, j) at Inv[i*64 + j] (zero outside the block's kb x kb)
+  T* const iv = Inv + d.offI + (long long)(t.k0 / 64) * 4096;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int idx = tid + 256 * q, i = idx >> 6, j = idx & 63;
+    iv[idx] = (i < kb && j < kb && i >= j) ? A[i * PM_LDA + j] : (T)0;
+  }
+}
+
 // Whole front with m <= 64 in one wavefront: lane i holds row i of the
 // m x m front [F1 .; F2 U] (panel columns from L, the update block from U,
 // which already holds the children's extend-adds).  The first s steps leave
@@ -1419,6 +1643,7 @@ __global__ __launch_bounds__(256) void k_extend_add_seq(const FrontDesc* __restr
   template __global__ void k_front_small<T>(const FrontDesc*, const int*, T*, T*, T*, int*); \
   template __global__ void k_potrf_diag<T>(const FrontDesc*, const PanelTask*, T*, T*, int*); \
   template __global__ void k_potrf_wave<T>(const FrontDesc*, const PanelTask*, T*, T*, int*, int); \
+  template __global__ void k_potrf_mw<T>(const FrontDesc*, const PanelTask*, T*, T*, int*, int);   \
   template __global__ void k_front_wave<T>(const FrontDesc*, const int*, T*, T*, T*, int*, int); \
   template __global__ void k_front_slot16<T>(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int); \
   template __global__ void k_front_slot32<T>(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int); \

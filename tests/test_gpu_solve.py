@@ -167,6 +167,50 @@ def test_lookahead_factor_is_bitwise_identical(width):
     assert np.array_equal(vals[0], vals[2])
 
 
+@pytest.mark.parametrize("prec", [T.FP64, T.FP32])
+def test_potrf_kernels_are_bitwise_identical(prec):
+    """The four-wave diagonal-block kernel (16-column panels, MFMA updates,
+    the inverse in LDS) and the one-wave kernel (register broadcasts) give
+    the same bits: the MFMA steps apply their products as fused
+    multiply-adds in k order, as the one-wave kernel does.  mesh3d 36 has
+    fronts of several 64-column blocks and partial last blocks; the dense
+    FP32 case takes the FP64 fallback in one block."""
+    cases = [("mesh3d 36", T.mesh3d(36), "genmmd")]
+    if prec == T.FP32:
+        M = np.zeros((128, 128))
+        M[:64, :64] = 4.0 * np.eye(64)
+        D = np.zeros((62, 64))
+        for i in range(62):
+            D[i, i:i + 3] = (1.0, -2.0, 1.0)
+        M[64:, 64:] = D.T @ D + 2.0 ** -20 * np.eye(64)
+        cp, ri, v = [0], [], []
+        for j in range(128):
+            for i in range(j, 128):
+                ri.append(i)
+                v.append(M[i, j])
+            cp.append(len(ri))
+        cases.append(("dense 128, flagged block",
+                      T.from_arrays(128, np.array(cp), np.array(ri), np.array(v)), "identity"))
+    for name, A, ordering in cases:
+        print(f"case: potrf kernels bitwise, {name}, precision {prec}", flush=True)
+        perm, inv = T.order(A, ordering)
+        PA = T.permute(A, perm, inv)
+        b = T.random_uniform(PA.n, 5)
+        res = []
+        try:
+            for kind in (2, 1):
+                T.lib.taucs_amd_set_potrf_kernel(kind)
+                F = T.factor_symbolic(PA)
+                assert F.numeric(PA, prec) == 0
+                res.append((F.digest(), F.solve(b), T.lib.taucs_amd_factor_fp64_fallbacks(F.h)))
+                F.free()
+        finally:
+            T.lib.taucs_amd_set_potrf_kernel(2)
+        assert res[0][0] == res[1][0]
+        assert np.array_equal(res[0][1], res[1][1])
+        assert res[0][2] == res[1][2]
+
+
 def test_matrix_file_through_the_drop_in(tmp_path):
     """The flow of the reference's direct driver (progs/direct.c:540-550) on a
     matrix read from a MatrixMarket file: order, factor on the GPU, solve with
