@@ -165,6 +165,11 @@ extern "C" void taucs_amd_set_panel_width(int w)
   g_panel_w = w < 64 ? 64 : (w > 1024 ? 1024 : w);
 }
 static int g_tile_big = 64;
+// With 128-tiles selected, a launch with fewer 128-tiles than this uses
+// 64-tiles instead (four times as many workgroups for the 256 CUs).
+#ifndef TILE128_MIN
+#define TILE128_MIN 512
+#endif
 extern "C" void taucs_amd_set_gemm_tile(int edge) { g_tile_big = edge == 128 ? 128 : 64; }
 static int g_lookahead = 1;
 extern "C" void taucs_amd_set_lookahead(int on) { g_lookahead = on; }
@@ -559,25 +564,41 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       }
       int g0 = (int)gemm.size();
       rest.clear();
+      // the tile edge of each launch: 128 only where the launch still has
+      // enough 128-tiles to fill the GPU, else 64
+      long long n_main = 0, n_rest = 0;
+      if (D->tile_big == 128)
+        for (int k = b0; k < b1; k++) {
+          const FrontDesc& d = fr[k];
+          if (d.m <= SMALL_M || os * W >= d.s) continue;
+          for (int c0 = os * W + W; c0 < d.s; c0 += 128)
+            (c0 < os * W + 2 * W ? n_main : n_rest) += (d.m - c0 + 127) / 128;
+        }
+      const int ts_main = D->tile_big == 128 && n_main >= TILE128_MIN ? 128 : 64;
+      const int ts_rest = D->tile_big == 128 && n_rest >= TILE128_MIN ? 128 : 64;
       for (int k = b0; k < b1; k++) {
         const FrontDesc& d = fr[k];
         if (d.m <= SMALL_M) continue;
         const int K0 = os * W;
         if (K0 >= d.s) continue;
-        const int kw = std::min(W, d.s - K0), ts = D->tile_big;
-        for (int c0 = K0 + W; c0 < d.s; c0 += ts)
-          for (int r0 = c0; r0 < d.m; r0 += ts)
-            (c0 < K0 + 2 * W ? gemm : rest).push_back(tile(d, r0, c0, K0, kw, ts));
+        const int kw = std::min(W, d.s - K0);
+        // main: the tiles that start within the next outer panel's columns
+        // (with W below the tile edge they reach beyond it); rest: the others
+        int c0 = K0 + W;
+        for (; c0 < d.s && c0 < K0 + 2 * W; c0 += ts_main)
+          for (int r0 = c0; r0 < d.m; r0 += ts_main) gemm.push_back(tile(d, r0, c0, K0, kw, ts_main));
+        for (; c0 < d.s; c0 += ts_rest)
+          for (int r0 = c0; r0 < d.m; r0 += ts_rest) rest.push_back(tile(d, r0, c0, K0, kw, ts_rest));
       }
       if ((int)gemm.size() > g0) {
         Op o = {TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l};
-        o.tile = D->tile_big;
+        o.tile = ts_main;
         D->ops.push_back(o);
       }
       if (!rest.empty()) {
         Op o = {TAMD_OP_GEMM, (int)gemm.size(), (int)rest.size(), l};
         o.lane = 1;
-        o.tile = D->tile_big;
+        o.tile = ts_rest;
         D->ops.push_back(o);
         gemm.insert(gemm.end(), rest.begin(), rest.end());
       }
@@ -585,11 +606,16 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     // SYRK of the update blocks, K = s
     {
       int g0 = (int)gemm.size();
+      long long n128 = 0;
+      for (int k = b0; k < b1; k++) {
+        const long long u = fr[k].m - fr[k].s;
+        if (fr[k].m > SMALL_M && u > 0) n128 += ((u + 127) / 128) * ((u + 127) / 128 + 1) / 2;
+      }
+      const int ts = D->tile_big == 128 && n128 >= TILE128_MIN ? 128 : 64;
       for (int k = b0; k < b1; k++) {
         const FrontDesc& d = fr[k];
         int u = d.m - d.s;
         if (d.m <= SMALL_M || u == 0) continue;
-        const int ts = D->tile_big;
         for (int j0 = 0; j0 < u; j0 += ts)
           for (int i0 = j0; i0 < u; i0 += ts) {
             GemmTask g;
@@ -608,7 +634,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       }
       if ((int)gemm.size() > g0) {
         Op o = {TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l};
-        o.tile = D->tile_big;
+        o.tile = ts;
         D->ops.push_back(o);
       }
     }

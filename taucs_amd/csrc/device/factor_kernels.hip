@@ -546,9 +546,9 @@ __global__ __launch_bounds__(256) void k_potrf_mw(const FrontDesc* __restrict__ 
   typedef typename Mfma<T>::acc_t acc_t;
   // L in the lower triangle; its inverse X, transposed, in the upper one
   // (X(r, c) at A[r*PM_LDA + c], r >= c; L's diagonal is kept in rdiag)
+  // (FP32: the FP64 fallback's packed triangle, 2080 doubles, takes the
+  // same 16,640 bytes)
   __shared__ __align__(16) T A[64 * PM_LDA];
-  constexpr int NPK = sizeof(T) == 4 ? 2080 : 1;  // FP32: the FP64 fallback's packed triangle
-  __shared__ __align__(16) double Pk[NPK];
   __shared__ __align__(16) T rdiag[64];
   __shared__ __align__(16) T dg[64];
   __shared__ __align__(16) T Ts[4][16 * 17];
@@ -624,7 +624,8 @@ __global__ __launch_bounds__(256) void k_potrf_mw(const FrontDesc* __restrict__ 
     __syncthreads();
     if (flagged) {
       if (wave == 0) {
-        double* P = Pk;
+        static_assert(sizeof(T) != 4 || 64 * PM_LDA * sizeof(T) >= 2080 * sizeof(double), "");
+        double* P = reinterpret_cast<double*>(A);  // the FP32 factor is discarded
         for (int c = 0; c <= lane; c++)
           P[ptri(lane, c)] = (lane < kb && c < kb) ? (double)gp[lane + c * m] : (lane == c ? 1.0 : 0.0);
         bool bad64 = false;
