@@ -512,8 +512,10 @@ def main():
     if side == 60 and os.path.exists(tp):
         with open(tp) as fh:
             tj = json.load(fh)
-        if tj.get("schedule") == sched:   # counters of this schedule's launches only
-            traffic = round(tj["bytes_per_launch"] / 1e6, 2)
+        # counters of this schedule's launches only
+        ent = tj.get("schedules", {}).get(f"{sched[0]}x{sched[1]}")
+        if ent:
+            traffic = round(ent["bytes_per_launch"] / 1e6, 2)
     gemm_rate = tm.gemm_flops / (tm.gemm_ms * 1e-3) if tm.gemm_ms > 0 else 0.0
     log(f"profile: total {tm.total_ms:.2f} ms, front {tm.front_ms:.2f} ms, gemm {tm.gemm_ms:.2f} ms "
         f"({tm.gemm_launches} launches, {gemm_rate/1e12:.2f} TFLOP/s), assembly {tm.assembly_ms:.2f} ms, "
@@ -592,9 +594,9 @@ def main():
         "rel_residual_after_1_refinement": info_ref.rel_residual,
         "mixed": mixed,
         "roofline": {"bound": "mfma",
-                     "kernel": ("tile GEMM/SYRK launches: k_gemm_tile128<double> (trailing updates, SYRK) "
-                                "and k_gemm_tile<double> (panel-internal updates), v_mfma_f64_16x16x4"
-                                if sched[1] == 128 else "k_gemm_tile<double> (v_mfma_f64_16x16x4)"),
+                     "kernel": ("tile GEMM/SYRK launches: k_gemm_tile128<double> and k_gemm_tile<double> "
+                                "(trailing updates, SYRKs; 128-tiles where a launch has at least 512 of them), "
+                                "k_gemm_short<double> (panel-internal updates, K = 64), v_mfma_f64_16x16x4"),
                      "achieved": round(gemm_rate / 1e12, 3), "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": round(gemm_rate / PEAK_FP64, 4),
                      "avg_launch_ms": round(gemm_avg_ms, 4), "launches": tm.gemm_launches,
