@@ -997,13 +997,16 @@ __global__ __launch_bounds__(64) void k_front_slot32(const FrontDesc* __restrict
                                                      int fallback)
 {
   typedef typename Mfma<T>::acc_t acc_t;
-  constexpr int NSB = 64 * PW_LD * sizeof(T) > 2080 * 8 ? 64 * PW_LD * sizeof(T) : 2080 * 8;
+  // each slot's 32 x 32 block in its own rows (row stride SL, columns local
+  // to the slot); FP32 also needs room for the FP64 fallback's triangle
+  constexpr int SL = 33;
+  constexpr int NSB = sizeof(T) == 4 ? 2080 * 8 : 64 * SL * (int)sizeof(T);
   __shared__ __align__(16) unsigned char sbuf[NSB];
   __shared__ __align__(16) T col[64];
   __shared__ __align__(16) T rdiag[64];
   __shared__ __align__(16) T dg[64];
   __shared__ __align__(16) T Ts[16 * 17];
-  T* Ls = reinterpret_cast<T*>(sbuf);  // 64 x PW_LD, columns indexed within the wave
+  T* Ls = reinterpret_cast<T*>(sbuf);  // row r, slot-local column c at Ls[r * SL + c]
   const SmallPack* pk = packs + blockIdx.x;
   const int lane = threadIdx.x, slot = lane >> 5, base = slot << 5, li = lane & 31;
   const int fi = pk->f[slot];
@@ -1022,7 +1025,7 @@ __global__ __launch_bounds__(64) void k_front_slot32(const FrontDesc* __restrict
   T* const gu = U + offU;
   // the lane's row of its front, staged through LDS (one load per entry at
   // run-time column offsets; the registers take it at fixed indices)
-  T* row = Ls + lane * PW_LD + base;
+  T* row = Ls + lane * SL;
 #pragma unroll
   for (int c = 0; c < 32; c++) row[c] = li == c ? (T)1 : (T)0;
   if (fi >= 0 && li < m) {
@@ -1125,27 +1128,27 @@ __global__ __launch_bounds__(64) void k_front_slot32(const FrontDesc* __restrict
   }
   wave_lds_fence();
 #pragma unroll
-  for (int c = 0; c < 32; c++) Ls[lane * PW_LD + base + c] = r[c];
+  for (int c = 0; c < 32; c++) Ls[lane * SL + c] = r[c];
   wave_lds_fence();
   const int b = lane >> 4, c16 = lane & 15, kq = lane >> 4;
   {
     T x[16];
-    const T* Lb = Ls + (16 * b) * PW_LD + 16 * b;
+    const T* Lb = Ls + (16 * b) * SL + 16 * (b & 1);
 #pragma clang loop unroll(full)
     for (int i = 0; i < 16; i++) {
       asm volatile("" ::: "memory");  // one row's entries in registers at a time
       T a0 = (i == c16) ? (T)1 : (T)0, a1 = (T)0;
 #pragma clang loop unroll(full)
       for (int t = 0; t < i; t += 2) {
-        a0 -= Lb[i * PW_LD + t] * x[t];
-        if (t + 1 < i) a1 -= Lb[i * PW_LD + t + 1] * x[t + 1];
+        a0 -= Lb[i * SL + t] * x[t];
+        if (t + 1 < i) a1 -= Lb[i * SL + t + 1] * x[t + 1];
       }
       x[i] = (a0 + a1) * rdiag[16 * b + i];
       asm volatile("" : "+v"(x[i]));
     }
 #pragma clang loop unroll(full)
     for (int i = 0; i < 16; i++)
-      if (i >= c16) Ls[(16 * b + c16) * PW_LD + 16 * b + i] = x[i];
+      if (i >= c16) Ls[(16 * b + c16) * SL + 16 * (b & 1) + i] = x[i];
   }
   wave_lds_fence();
   // each front's off-diagonal block: X_10 = -X_11 (L_10 X_00), pairs (1,0), (3,2)
@@ -1158,8 +1161,8 @@ __global__ __launch_bounds__(64) void k_front_slot32(const FrontDesc* __restrict
 #pragma clang loop unroll(full)
     for (int st = 0; st < 4; st++) {
       const int kr = 4 * st + kq;
-      const T a = Ls[(16 * i + c16) * PW_LD + 16 * j + kr];
-      const T xv = kr >= c16 ? Ls[(16 * j + c16) * PW_LD + 16 * j + kr] : (T)0;
+      const T a = Ls[(16 * i + c16) * SL + 16 * (j & 1) + kr];
+      const T xv = kr >= c16 ? Ls[(16 * j + c16) * SL + 16 * (j & 1) + kr] : (T)0;
       acc = Mfma<T>::step(a, xv, acc);
     }
 #pragma clang loop unroll(full)
@@ -1171,19 +1174,19 @@ __global__ __launch_bounds__(64) void k_front_slot32(const FrontDesc* __restrict
 #pragma clang loop unroll(full)
     for (int st = 0; st < 4; st++) {
       const int kr = 4 * st + kq;
-      const T a = (c16 >= kr) ? Ls[(16 * i + kr) * PW_LD + 16 * i + c16] : (T)0;
+      const T a = (c16 >= kr) ? Ls[(16 * i + kr) * SL + 16 * (i & 1) + c16] : (T)0;
       xr = Mfma<T>::step(a, Ts[kr * 17 + c16], xr);
     }
 #pragma clang loop unroll(full)
     for (int e = 0; e < 4; e++)
-      Ls[(16 * j + c16) * PW_LD + 16 * i + Mfma<T>::drow(lane, e)] = -xr[e];
+      Ls[(16 * j + c16) * SL + 16 * (i & 1) + Mfma<T>::drow(lane, e)] = -xr[e];
     wave_lds_fence();
   }
   // lane holds column `lane` of its front's inverse: X(base + i, lane)
   if (fi >= 0 && li < s) {
     T* iv = Inv + offI;
     for (int i = 0; i < s; i++)
-      iv[i * 64 + li] = i >= li ? Ls[lane * PW_LD + base + i] : (T)0;
+      iv[i * 64 + li] = i >= li ? Ls[lane * SL + i] : (T)0;
   }
   if (bad) atomicOr(info, 1);
 }
