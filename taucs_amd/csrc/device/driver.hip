@@ -1271,7 +1271,7 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
           EMIT(lane, join, k_gemm_tile_bf16, dim3(op.count), dim3(256),
                (const GemmTask*)(D->d_gemm + op.start), (const float*)L, (float*)L, (float*)U);
         else if (op.tile == 128)
-          EMIT(lane, join, k_gemm_tile128<T>, dim3(op.count), dim3(256),
+          EMIT(lane, join, k_gemm_tile128<T>, dim3(op.count), dim3(512),
                (const GemmTask*)(D->d_gemm + op.start), (const T*)L, L, U);
         else if (op.shortk && lane == 0)
           EMIT(lane, join, k_gemm_short<T>, dim3(op.count), dim3(256),

