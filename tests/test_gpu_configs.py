@@ -160,6 +160,28 @@ def test_c2_lookahead_factor_is_bitwise_the_serial_one(c2, sched):
     assert dig[0] == dig[1] == dig[2], dig
 
 
+def test_c2_ccs_solve_at_full_size(c2):
+    """taucs_ccs_solve_llt on the exported 60^3 factor (nnz(L) 1.5e8, the size at
+    which an earlier per-value wait bound of the sync-free sweeps was reached):
+    both sweeps finish, and the solution equals the supernodal solve's to
+    rounding (the two sum each row in different orders)."""
+    _, PA, B, M = c2
+    print("case: mesh3d 60^3 ccs solve of the exported factor", flush=True)
+    F = T.factor_symbolic(PA)
+    assert F.numeric(PA, T.FP64) == 0
+    b = np.ascontiguousarray(B[:, 0])
+    x_sn = F.solve(b)
+    Lc = F.to_ccs()
+    F.free()
+    x_ccs = Lc.solve_llt(b)
+    d = np.linalg.norm(x_ccs - x_sn) / np.linalg.norm(x_sn)
+    rr = np.linalg.norm(M @ x_ccs - b) / np.linalg.norm(b)
+    print(f"  ccs vs supernodal {d:.2e}, relres {rr:.2e}", flush=True)
+    # cond ~ 4.4e6: two backward-stable solves agree to ~ cond * u
+    assert d <= 1e-9
+    assert rr <= 1e-11
+
+
 @pytest.fixture(scope="module")
 def c3():
     print("case: mesh3d_varcoef 80^3 setup (genmmd, symbolic)", flush=True)
