@@ -184,8 +184,9 @@ void  taucs_amd_set_lookahead(int on);
    through LDS.  2 and 1 give bitwise equal factors.  Applies to numeric
    factorizations started afterwards. */
 void  taucs_amd_set_potrf_kernel(int kind);
-/* Fronts with at most 16 (32) rows are factored four (two) to a wavefront
-   (default on); the results are bitwise those of one front per wavefront.
+/* Fronts with at most 16 (32) rows are factored four (two) to a wavefront,
+   and fronts with at most 48 rows with 48-wide staging (default on); the
+   results are bitwise those of the 64-row one-front-per-wavefront kernel.
    Applies to schedules built afterwards. */
 void  taucs_amd_set_front_packing(int on);
 /* Triangular sweeps of the solve: 1 = one launch per tree level, the block

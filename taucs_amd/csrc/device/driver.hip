@@ -27,6 +27,7 @@ template <typename T> __global__ void k_potrf_mw(const FrontDesc*, const PanelTa
 template <typename T> __global__ void k_front_wave(const FrontDesc*, const int*, T*, T*, T*, int*, int);
 template <typename T> __global__ void k_front_slot16(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int);
 template <typename T> __global__ void k_front_slot32(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int);
+template <typename T> __global__ void k_front_w48(const FrontDesc*, const int*, T*, T*, T*, int*, int);
 template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*, const T*);
 template <typename T> __global__ void k_trsm_tile(const GemmTask*, const T*, T*, T*, const T*);
 template <typename T> __global__ void k_gemm_tile128(const GemmTask*, const T*, T*, T*);
@@ -179,13 +180,15 @@ extern "C" void taucs_amd_set_lookahead(int on) { g_lookahead = on; }
 static int g_potrf_wave = 2;
 extern "C" void taucs_amd_set_potrf_kernel(int kind) { g_potrf_wave = kind >= 0 && kind <= 2 ? kind : 2; }
 // Small fronts several to a wavefront: m <= 16 four per wave (k_front_slot16),
-// m <= 32 two per wave (k_front_slot32); results are bitwise those of one
-// front per wave.  Applies to schedules built afterwards (default on).
+// m <= 32 two per wave (k_front_slot32), and m <= 48 one per wave with
+// 48-wide staging (k_front_w48, half the LDS of k_front_wave); results are
+// bitwise those of k_front_wave.  Applies to schedules built afterwards
+// (default on).
 static int g_front_packing = 1;
 extern "C" void taucs_amd_set_front_packing(int on) { g_front_packing = on ? 1 : 0; }
 // Reduced-precision factors: a diagonal block whose FP32 Cholesky meets a
 // nonpositive pivot or a pivot below 2^-12 of its input diagonal entry is
-// refactored in FP64 (the one-wave POTRF kernels; default on).
+// refactored in FP64 (the diagonal-block and whole-front kernels; default on).
 static int g_fp64_fallback = 1;
 extern "C" void taucs_amd_set_fp64_fallback(int on) { g_fp64_fallback = on ? 1 : 0; }
 
@@ -466,10 +469,11 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     int st = (int)small.size();
     int maxouter = 0;
     const int W = D->panel_w;
-    std::vector<int> c16, c32;
+    std::vector<int> c16, c32, c48;
     for (int k = b0; k < b1; k++) {
       if (g_front_packing && fr[k].m <= 16) c16.push_back(k);
       else if (g_front_packing && fr[k].m <= 32) c32.push_back(k);
+      else if (g_front_packing && fr[k].m <= 48) c48.push_back(k);
       else if (fr[k].m <= SMALL_M) small.push_back(k);
       else maxouter = std::max(maxouter, (fr[k].s + W - 1) / W);
     }
@@ -495,6 +499,13 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
       Op o = {TAMD_OP_SMALL, st, (int)small.size() - st, l};
       o.nojoin = true;
       D->ops.push_back(o);
+    }
+    if (!c48.empty()) {  // one front per wavefront, 48-wide staging
+      Op o = {TAMD_OP_SMALL, (int)small.size(), (int)c48.size(), l};
+      o.nojoin = true;
+      o.tile = 48;
+      D->ops.push_back(o);
+      small.insert(small.end(), c48.begin(), c48.end());
     }
     // Two-level blocking: outer panels of W columns, each factored in 64-column
     // steps (POTRF of the diagonal block, TRSM of the rows below by its
@@ -1244,6 +1255,9 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
         else if (op.tile == 32)
           EMIT(lane, join, k_front_slot32<T>, dim3(op.count), dim3(64), (const FrontDesc*)D->d_fronts,
                (const SmallPack*)(D->d_packs + op.start), L, U, Iv, D->d_info, g_fp64_fallback);
+        else if (op.tile == 48)
+          EMIT(lane, join, k_front_w48<T>, dim3(op.count), dim3(64), (const FrontDesc*)D->d_fronts,
+               (const int*)(D->d_small + op.start), L, U, Iv, D->d_info, g_fp64_fallback);
         else if (g_potrf_wave)
           EMIT(lane, join, k_front_wave<T>, dim3(op.count), dim3(64), (const FrontDesc*)D->d_fronts,
                (const int*)(D->d_small + op.start), L, U, Iv, D->d_info, g_fp64_fallback);

@@ -1191,6 +1191,205 @@ __global__ __launch_bounds__(64) void k_front_slot32(const FrontDesc* __restrict
   if (bad) atomicOr(info, 1);
 }
 
+// Fronts with 32 < m <= 48, one to a wavefront: each lane holds 48 columns of
+// its row and the LDS staging is 48 wide, 20 KB in FP64 against the 37 KB of
+// the 64-row kernel, so twice as many waves fit on a CU.  Lanes 48..63 carry
+// no row.  Every operation on the front is the one k_front_wave applies to it
+// (pivot by lane broadcast and sqrt_pair for the first s columns, rank-1
+// updates in column order, the FP64 fallback over the packed 64-row
+// triangle, the diagonal 16 x 16 substitutions and the MFMA products of the
+// block pairs (1,0), (2,1), (2,0); the pairs with block 3 only meet its
+// identity padding there), so the results are bitwise those of k_front_wave.
+template <typename T>
+__global__ __launch_bounds__(64) void k_front_w48(const FrontDesc* __restrict__ fr,
+                                                  const int* __restrict__ list,
+                                                  T* __restrict__ L, T* __restrict__ U,
+                                                  T* __restrict__ Inv, int* __restrict__ info,
+                                                  int fallback)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int SL = 49;
+  constexpr int NSB = sizeof(T) == 4 ? 2080 * 8 : 48 * SL * (int)sizeof(T);
+  __shared__ __align__(16) unsigned char sbuf[NSB];
+  __shared__ __align__(16) T col[64];
+  __shared__ __align__(16) T rdiag[64];
+  __shared__ __align__(16) T dg[64];
+  __shared__ __align__(16) T Ts[16 * 17];
+  T* Ls = reinterpret_cast<T*>(sbuf);  // row r < 48, column c < 48 at Ls[r * SL + c]
+  const FrontDesc d = fr[list[blockIdx.x]];
+  const int s = d.s, m = d.m, u = m - s, lane = threadIdx.x;
+  const bool act = lane < 48;
+  T* const gp = L + d.offL;
+  T* const gu = U + d.offU;
+  T* row = Ls + (act ? lane : 0) * SL;
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < 48; c++) row[c] = lane == c ? (T)1 : (T)0;
+    if (lane < m) {
+      for (int c = 0; c <= lane; c++) {
+        if (c < s) row[c] = gp[lane + (long long)c * m];
+        else row[c] = gu[(lane - s) + (long long)(c - s) * u];
+      }
+    }
+  }
+  wave_lds_fence();
+  T r[48];
+#pragma unroll
+  for (int c = 0; c < 48; c++) r[c] = act ? row[c] : (T)0;
+  if constexpr (sizeof(T) == 4) {
+    T v = (T)1;
+#pragma unroll
+    for (int c = 0; c < 48; c++)
+      if (c == lane) v = r[c];
+    dg[lane] = v;
+    wave_lds_fence();
+  }
+  bool bad = false, grow = false;
+#pragma clang loop unroll(full)
+  for (int k = 0; k < 48; k++) {
+    if (k < s) {
+      T p = rlane(r[k], k);
+      if constexpr (sizeof(T) == 4) {
+        if (p < (T)GROW_TAU * dg[k]) grow = true;
+      }
+      if (!(p > (T)0)) { bad = true; p = (T)1; }
+      T dk, rdk;
+      sqrt_pair(p, dk, rdk);
+      const T lik = r[k] * rdk;
+      r[k] = lane == k ? dk : lik;
+      col[lane] = r[k];
+      rdiag[k] = rdk;
+      wave_lds_fence();
+#pragma clang loop unroll(full)
+      for (int j0 = k + 1; j0 < 48; j0 += 16) {
+        T v[16];
+#pragma clang loop unroll(full)
+        for (int q = 0; q < 16; q++)
+          if (j0 + q < 48) v[q] = col[j0 + q];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma clang loop unroll(full)
+        for (int q = 0; q < 16; q++)
+          if (j0 + q < 48) r[j0 + q] -= lik * v[q];
+      }
+#pragma clang loop unroll(full)
+      for (int j = k + 1; j < 48; j++) asm volatile("" : "+v"(r[j]));  // as in wave_chol_inv
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (fallback && (bad || grow)) {  // wave-uniform (pivots come by readlane)
+      double* P = reinterpret_cast<double*>(sbuf);
+      const unsigned long long piv = (1ULL << s) - 1;
+      for (int c = 0; c <= lane; c++) {
+        double v = lane == c ? 1.0 : 0.0;
+        if (c < s) {
+          if (lane < m) v = (double)gp[lane + (long long)c * m];
+        } else if (c < m) {
+          if (lane < m) v = (double)gu[(lane - s) + (long long)(c - s) * u];
+        }
+        P[ptri(lane, c)] = v;
+      }
+      bool bad64 = false;
+      wave_chol_packed_f64(P, lane, piv, bad64);
+#pragma unroll
+      for (int c = 0; c < 48; c++) r[c] = c <= lane ? (T)P[ptri(lane, c)] : (T)0;
+      rdiag[lane] = ((piv >> lane) & 1ULL) ? (T)(1.0 / P[ptri(lane, lane)]) : (T)1;
+      wave_lds_fence();
+      bad = bad64;
+      if (lane == 0) atomicAdd(info + 1, 1);
+    }
+  }
+  // the factored row back through LDS: L in the panel columns, the Schur
+  // complement in the update block
+  wave_lds_fence();
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < 48; c++) row[c] = r[c];
+  }
+  wave_lds_fence();
+  if (act && lane < m) {
+    for (int c = 0; c <= lane; c++) {
+      if (c < s) gp[lane + (long long)c * m] = row[c];
+      else guw_store(gu, lane, s, c, u, row[c]);
+    }
+  }
+  // inverse of the leading s x s block (its other rows: identity)
+  if (lane >= s) {
+#pragma unroll
+    for (int c = 0; c < 48; c++) r[c] = c == lane ? (T)1 : (T)0;
+    rdiag[lane] = (T)1;
+  }
+  wave_lds_fence();
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < 48; c++) Ls[lane * SL + c] = r[c];
+  }
+  wave_lds_fence();
+  const int b = lane >> 4, c16 = lane & 15, kq = lane >> 4;
+  if (b < 3) {
+    T x[16];
+    const T* Lb = Ls + (16 * b) * SL + 16 * b;
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 16; i++) {
+      asm volatile("" ::: "memory");  // one row's entries in registers at a time
+      T a0 = (i == c16) ? (T)1 : (T)0, a1 = (T)0;
+#pragma clang loop unroll(full)
+      for (int t = 0; t < i; t += 2) {
+        a0 -= Lb[i * SL + t] * x[t];
+        if (t + 1 < i) a1 -= Lb[i * SL + t + 1] * x[t + 1];
+      }
+      x[i] = (a0 + a1) * rdiag[16 * b + i];
+      asm volatile("" : "+v"(x[i]));
+    }
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 16; i++)
+      if (i >= c16) Ls[(16 * b + c16) * SL + 16 * b + i] = x[i];
+  }
+  wave_lds_fence();
+  // off-diagonal blocks by distance: X_ij = -X_ii (sum_{k=j}^{i-1} L_ik X_kj)
+#pragma clang loop unroll(full)
+  for (int dd = 1; dd < 3; dd++) {
+#pragma clang loop unroll(full)
+    for (int j = 0; j + dd < 3; j++) {
+      const int i = j + dd;
+      acc_t acc;
+#pragma clang loop unroll(full)
+      for (int e = 0; e < 4; e++) acc[e] = (T)0;
+#pragma clang loop unroll(full)
+      for (int k = j; k < i; k++)
+#pragma clang loop unroll(full)
+        for (int st = 0; st < 4; st++) {
+          const int kr = 4 * st + kq;
+          const T a = Ls[(16 * i + c16) * SL + 16 * k + kr];
+          const T xv = (k > j || kr >= c16) ? Ls[(16 * j + c16) * SL + 16 * k + kr] : (T)0;
+          acc = Mfma<T>::step(a, xv, acc);
+        }
+#pragma clang loop unroll(full)
+      for (int e = 0; e < 4; e++) Ts[Mfma<T>::drow(lane, e) * 17 + c16] = acc[e];
+      wave_lds_fence();
+      acc_t xr;
+#pragma clang loop unroll(full)
+      for (int e = 0; e < 4; e++) xr[e] = (T)0;
+#pragma clang loop unroll(full)
+      for (int st = 0; st < 4; st++) {
+        const int kr = 4 * st + kq;
+        const T a = (c16 >= kr) ? Ls[(16 * i + kr) * SL + 16 * i + c16] : (T)0;
+        xr = Mfma<T>::step(a, Ts[kr * 17 + c16], xr);
+      }
+#pragma clang loop unroll(full)
+      for (int e = 0; e < 4; e++)
+        Ls[(16 * j + c16) * SL + 16 * i + Mfma<T>::drow(lane, e)] = -xr[e];
+      wave_lds_fence();
+    }
+  }
+  // the whole 64 x 64 inverse block: X(i, lane) in the transposed upper
+  // triangle, zero outside the leading s x s
+  T* iv = Inv + d.offI;
+  for (int i = 0; i < 64; i++)
+    iv[i * 64 + lane] = (lane < s && i < s && i >= lane) ? Ls[lane * SL + i] : (T)0;
+  if (lane == 0 && bad) atomicOr(info, 1);
+}
+
 // ---- MFMA tile GEMM ---------------------------------------------------------
 // 64x64 tile, 4 waves (2x2), each wave a 32x32 sub-tile of 2x2 16x16 MFMA
 // blocks.  K is staged through LDS in chunks of 16.  The MFMA is issued as
@@ -1654,6 +1853,7 @@ __global__ __launch_bounds__(256) void k_extend_add_seq(const FrontDesc* __restr
   template __global__ void k_front_wave<T>(const FrontDesc*, const int*, T*, T*, T*, int*, int); \
   template __global__ void k_front_slot16<T>(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int); \
   template __global__ void k_front_slot32<T>(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int); \
+  template __global__ void k_front_w48<T>(const FrontDesc*, const int*, T*, T*, T*, int*, int);          \
   template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_trsm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_gemm_tile128<T>(const GemmTask*, const T*, T*, T*);          \
