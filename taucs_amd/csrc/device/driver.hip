@@ -20,6 +20,7 @@ namespace tamd {
 template <typename T> __global__ void k_fill(T*, long long);
 template <typename T> __global__ void k_fill_ranges(const ZeroTask*, T*);
 template <typename T> __global__ void k_assemble(int, const int*, const long long*, const double*, T*);
+template <typename T> __global__ void k_assemble_flat(long long, const long long*, const double*, T*);
 template <typename T> __global__ void k_front_small(const FrontDesc*, const int*, T*, T*, T*, int*);
 template <typename T> __global__ void k_potrf_diag(const FrontDesc*, const PanelTask*, T*, T*, int*);
 template <typename T> __global__ void k_potrf_wave(const FrontDesc*, const PanelTask*, T*, T*, int*, int);
@@ -93,6 +94,7 @@ struct tamd_dev {
   int* d_rows = nullptr;
   int* d_rel = nullptr;
   int* d_colptr = nullptr;
+  bool assemble_flat = false;  // every A entry has its own destination in L
   long long* d_adest = nullptr;
   int* d_small = nullptr;        // front indices for k_front_small / k_front_wave
   SmallPack* d_packs = nullptr;  // fronts packed 4 (m <= 16) or 2 (m <= 32) to a wave
@@ -954,6 +956,27 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     }
   }
   D->ncol = (int)ncol;
+  // A's entries can only share a destination within one column (a duplicated
+  // (row, column) pair); without one, assembly runs one thread per entry
+  {
+    bool dup = false;
+    std::vector<long long> cd;
+    for (long long j = 0; j < ncol && !dup; j++) {
+      const long long* c = adest.data() + colptr[j];
+      const int cnt = colptr[j + 1] - colptr[j];
+      if (cnt <= 16) {  // short columns pairwise, in place
+        for (int a = 1; a < cnt && !dup; a++)
+          for (int b = 0; b < a; b++)
+            if (c[a] == c[b]) dup = true;
+      } else {
+        cd.assign(c, c + cnt);
+        std::sort(cd.begin(), cd.end());
+        for (size_t q = 1; q < cd.size(); q++)
+          if (cd[q] == cd[q - 1]) dup = true;
+      }
+    }
+    D->assemble_flat = !dup;
+  }
   // GEMM launches of 64-tiles with K <= 64 use the one-round-trip tile
   for (Op& op : D->ops)
     if (op.kind == TAMD_OP_GEMM && op.tile == 64) {
@@ -1245,8 +1268,12 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
              (const ZeroTask*)(D->d_zero + op.start), U);
         break;
       case TAMD_OP_ASSEMBLE:
-        EMIT(lane, join, k_assemble<T>, dim3(grid_for(D->ncol, 128)), dim3(128), D->ncol,
-             (const int*)D->d_colptr, (const long long*)D->d_adest, (const double*)D->d_avals, L);
+        if (D->assemble_flat)
+          EMIT(lane, join, k_assemble_flat<T>, dim3(grid_for(D->nnz, 256)), dim3(256),
+               (long long)D->nnz, (const long long*)D->d_adest, (const double*)D->d_avals, L);
+        else
+          EMIT(lane, join, k_assemble<T>, dim3(grid_for(D->ncol, 128)), dim3(128), D->ncol,
+               (const int*)D->d_colptr, (const long long*)D->d_adest, (const double*)D->d_avals, L);
         break;
       case TAMD_OP_SMALL:
         if (op.tile == 16)

@@ -38,6 +38,19 @@ __global__ void k_assemble(int ncol, const int* __restrict__ colptr,
   for (int ip = colptr[j]; ip < e; ip++) L[dest[ip]] += (T)vals[ip];
 }
 
+// One thread per A entry, for patterns whose entries all have distinct
+// destinations (the host checks when it builds the schedule): the same single
+// addition per destination as k_assemble, without its chain of dependent
+// read-modify-writes down each column.
+template <typename T>
+__global__ void k_assemble_flat(long long nnz, const long long* __restrict__ dest,
+                                const double* __restrict__ vals, T* __restrict__ L)
+{
+  const long long step = (long long)gridDim.x * blockDim.x;
+  for (long long ip = (long long)blockIdx.x * blockDim.x + threadIdx.x; ip < nnz; ip += step)
+    L[dest[ip]] += (T)vals[ip];
+}
+
 // Right-looking unblocked Cholesky of an nrows x ncols panel (both <= 64),
 // fused with the inversion of the factored diagonal block (X = L11^{-1}).
 // The panel is staged through LDS into registers: thread (wave w, lane j)
@@ -1844,6 +1857,7 @@ __global__ __launch_bounds__(256) void k_extend_add_seq(const FrontDesc* __restr
 #define TAMD_INST(T)                                                                   \
   template __global__ void k_fill<T>(T*, long long);                                   \
   template __global__ void k_fill_ranges<T>(const ZeroTask*, T*);                       \
+  template __global__ void k_assemble_flat<T>(long long, const long long*, const double*, T*); \
   template __global__ void k_assemble<T>(int, const int*, const long long*,             \
                                          const double*, T*);                            \
   template __global__ void k_front_small<T>(const FrontDesc*, const int*, T*, T*, T*, int*); \

@@ -171,6 +171,35 @@ def test_symbolic_once_numeric_many():
     assert np.array_equal(F.solve(b), x1)
 
 
+def test_duplicate_entries_are_summed_in_stored_order():
+    """A pattern with a duplicated (row, column) pair assembles column by
+    column, adding the duplicates in stored order as the reference adds every
+    stored entry into its front; a pattern without one assembles one entry
+    per thread.  Splitting an off-diagonal entry v into v/2 + v/2 (exact in
+    binary) must give bitwise the factor and solution of the original."""
+    print("case: duplicated entry, 3d 8", flush=True)
+    A = T.mesh3d(8)
+    _, _, PA = prepared(A)
+    n, cp, ri, v = PA.arrays()
+    j = next(c for c in range(n) if any(ri[q] != c for q in range(cp[c], cp[c + 1])))
+    k = next(q for q in range(cp[j], cp[j + 1]) if ri[q] != j)
+    cp2 = cp.copy()
+    cp2[j + 1:] += 1
+    ri2 = np.insert(ri, k + 1, ri[k])
+    v2 = np.insert(v, k + 1, v[k] / 2)
+    v2[k] = v[k] / 2
+    D = T.from_arrays(n, cp2, ri2, v2)
+    F1 = T.factor_symbolic(PA)
+    assert F1.numeric(PA) == 0
+    F2 = T.factor_symbolic(D)
+    assert F2.numeric(D) == 0
+    assert F1.digest() == F2.digest()
+    b = T.random_uniform(n, 3)
+    assert np.array_equal(F1.solve(b), F2.solve(b))
+    F1.free()
+    F2.free()
+
+
 def test_results_are_deterministic():
     print("case: determinism 3d 24", flush=True)
     A = T.mesh3d(24)
