@@ -441,10 +441,18 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
         int sn = P->order[k];
         long long u = S->sn_up_size[sn] - S->sn_size[sn];
         if (u <= 0) continue;
-        long long off = ub + P->off_U[sn], len = u * u;
+        const long long off = ub + P->off_U[sn];
         auto& za = (P->u_start[sn] == P->sn_level[sn] ? zero_at : zero_par_at)[P->u_start[sn]];
-        for (long long c = 0; c < len; c += 65536)
-          za.push_back({off + c, std::min(65536LL, len - c)});
+        // the lower triangle, by column ranges of about 64K elements
+        long long cnt = 0;
+        for (int c = 0, c0 = 0; c < (int)u; c++) {
+          cnt += u - c;
+          if (cnt >= 65536 || c == (int)u - 1) {
+            za.push_back({off, (int)u, c0, c + 1, 0});
+            c0 = c + 1;
+            cnt = 0;
+          }
+        }
       }
       ub += P->U_elems;
     }
@@ -1009,7 +1017,10 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     else if (op.kind == TAMD_OP_ASSEMBLE)
       b = 20.0 * (double)nnz_tot;
     else if (op.kind == TAMD_OP_ZERO)
-      for (int q = op.start; q < op.start + op.count; q++) b += 8.0 * (double)zero[q].len;
+      for (int q = op.start; q < op.start + op.count; q++) {
+        const double u = zero[q].u, c0 = zero[q].c0, c1 = zero[q].c1;
+        b += 8.0 * ((c1 - c0) * u - 0.5 * (c1 * (c1 - 1) - c0 * (c0 - 1)));
+      }
     D->op_flops[o] = f;
     D->op_bytes[o] = b;
   }

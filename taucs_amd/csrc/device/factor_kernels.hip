@@ -16,13 +16,17 @@ __global__ void k_fill(T* __restrict__ p, long long n)
   for (; i < n; i += stride) p[i] = (T)0;
 }
 
-// Zero the update blocks whose life starts at this level (one task = one
-// block range of at most 64K elements).
+// Zero the update blocks whose life starts at this level, lower triangles
+// only (one task = a column range of at most 64K elements): every reader of
+// an update block (the extend-add, the SYRK epilogue, the whole-front
+// kernels) touches entries (i, j) with i >= j alone.
 template <typename T>
 __global__ void k_fill_ranges(const ZeroTask* __restrict__ tasks, T* __restrict__ U)
 {
   const ZeroTask t = tasks[blockIdx.x];
-  for (long long i = threadIdx.x; i < t.len; i += blockDim.x) U[t.off + i] = (T)0;
+  T* const b = U + t.off;
+  for (int c = t.c0; c < t.c1; c++)
+    for (int i = c + threadIdx.x; i < t.u; i += blockDim.x) b[i + (long long)c * t.u] = (T)0;
 }
 
 // One thread per A column, entries in stored order: duplicates (if any) are
@@ -762,8 +766,8 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
     T v = lane == c ? (T)1 : (T)0;
     if (c < s) {
       if (lane < m) v = gp[lane + c * m];
-    } else if (c < m) {
-      if (lane >= s && lane < m) v = gu[(lane - s) + (c - s) * u];
+    } else if (c < m) {  // the update block's lower triangle (the rest is never written)
+      if (lane >= c && lane < m) v = gu[(lane - s) + (c - s) * u];
     }
     r[c] = v;
   }

@@ -55,8 +55,9 @@ struct SmallPack {  // fronts with m <= width, one per `width`-lane slot of a wa
   int pad[3];
 };
 
-struct ZeroTask {   // U[off .. off+len) <- 0
-  long long off, len;
+struct ZeroTask {   // rows c..u-1 of columns c0..c1-1 of the u x u block at U[off] <- 0
+  long long off;
+  int u, c0, c1, pad;
 };
 
 struct EATask {     // extend-add columns j0..j1-1 of a child's update block
