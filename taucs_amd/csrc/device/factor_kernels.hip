@@ -1801,6 +1801,34 @@ __device__ __forceinline__ void extend_add_cols(const FrontDesc* __restrict__ fr
   const T* Uc = U + c.offU;
   const int* r = rel + c.rel;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (t.j1 - t.j0 < 4) {
+    // fewer columns than waves (the long columns of a large child): all 256
+    // threads walk each column, four rows a thread per pass, loads first
+    for (int j = t.j0; j < t.j1; j++) {
+      const int pj = r[j];
+      const T* src = Uc + (long long)j * uc;
+      T* base = pj < sp ? L : U;
+      const long long col0 = pj < sp ? p.offL + (long long)pj * mp
+                                     : p.offU + (long long)(pj - sp) * up - sp;
+      for (int i0 = j + (int)threadIdx.x; i0 < uc; i0 += 1024) {
+        T sv[4], bv[4];
+        int ri[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int i = i0 + 256 * q;
+          sv[q] = i < uc ? src[i] : (T)0;
+          ri[q] = i < uc ? r[i] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (i0 + 256 * q < uc) bv[q] = base[col0 + ri[q]];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (i0 + 256 * q < uc) base[col0 + ri[q]] = bv[q] + sv[q];
+      }
+    }
+    return;
+  }
   for (int j = t.j0 + wave; j < t.j1; j += 4) {
     const int pj = r[j];
     const T* src = Uc + (long long)j * uc;
