@@ -236,6 +236,11 @@ int  taucs_amd_batch_get_timing(void* B, taucs_amd_numeric_timing* t);
    Returns the number of ops (arrays hold up to cap entries). */
 int  taucs_amd_get_op_profile(void* L, int cap, int* kind, int* level, int* count,
                               double* flops, double* bytes, double* ms);
+/* The tile shape of the same ops: the tile edge of a GEMM op (64 or 128; 0
+   for other kinds) and whether its tasks all have K <= 64 (the panel-internal
+   updates, k_gemm_short).  Returns the op count, or -1 without a device
+   schedule.  No counterpart in the reference (inspection). */
+int  taucs_amd_get_op_tiles(void* L, int cap, int* tile, int* shortk);
 
 #ifdef __cplusplus
 }

@@ -130,6 +130,7 @@ _sigs = {
     "taucs_amd_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                            C.c_void_p, C.c_void_p]),
     "taucs_amd_get_op_profile": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 6),
+    "taucs_amd_get_op_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "taucs_amd_batch_factor_llt_mf": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int]),
     "taucs_amd_batch_numeric": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "taucs_amd_batch_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
@@ -474,7 +475,8 @@ class Factor:
         return cand[best.value], dict(zip(cand, ms.tolist()))
 
     def op_profile(self):
-        """Per-op kind, level, tasks, flops, bytes and ms of the last profiled numeric run."""
+        """Per-op kind, level, tasks, flops, bytes and ms of the last profiled numeric run, and
+        the tile edge and K <= 64 flag of the GEMM ops."""
         n = lib.taucs_amd_get_op_profile(self.h, 0, None, None, None, None, None, None)
         if n < 0:
             raise RuntimeError("no device schedule")
@@ -482,7 +484,10 @@ class Factor:
         f, b, ms = (np.zeros(n) for _ in range(3))
         lib.taucs_amd_get_op_profile(self.h, n, _dptr(k), _dptr(lv), _dptr(c), _dptr(f), _dptr(b),
                                      _dptr(ms))
-        return {"kind": k, "level": lv, "count": c, "flops": f, "bytes": b, "ms": ms}
+        tile, shortk = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+        lib.taucs_amd_get_op_tiles(self.h, n, _dptr(tile), _dptr(shortk))
+        return {"kind": k, "level": lv, "count": c, "flops": f, "bytes": b, "ms": ms, "tile": tile,
+                "shortk": shortk}
 
     def free(self):
         if self.h:

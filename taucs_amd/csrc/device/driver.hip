@@ -1537,6 +1537,20 @@ extern "C" int taucs_amd_get_op_profile(void* vL, int cap, int* kind, int* level
   return n;
 }
 
+extern "C" int taucs_amd_get_op_tiles(void* vL, int cap, int* tile, int* shortk)
+{
+  tamd_factor* F = (tamd_factor*)vL;
+  if (!F || !F->dev) return -1;
+  tamd_dev* D = F->dev;
+  const int n = (int)D->ops.size();
+  for (int o = 0; o < n && o < cap; o++) {
+    const bool g = D->ops[o].kind == TAMD_OP_GEMM;
+    if (tile) tile[o] = g ? D->ops[o].tile : 0;
+    if (shortk) shortk[o] = g && D->ops[o].shortk ? 1 : 0;
+  }
+  return n;
+}
+
 extern "C" int taucs_amd_factor_fp64_fallbacks(void* vL)
 {
   tamd_factor* F = (tamd_factor*)vL;
