@@ -1868,6 +1868,45 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict_
   extend_add_cols(fr, tasks[blockIdx.x], rel, L, U);
 }
 
+// A whole child whose u x u block has at most 2048 entries, in one pass of
+// the workgroup: the block's entries as one flat index, eight a thread, the
+// lower ones (i >= j) added into the parent with all loads before the stores
+// (the entries go to distinct parent positions).
+#define EA_FLAT_MAX 2048
+template <typename T>
+__device__ __forceinline__ void extend_add_flat(const FrontDesc* __restrict__ fr, const EATask& t,
+                                                const int* __restrict__ rel, T* __restrict__ L,
+                                                T* __restrict__ U)
+{
+  const FrontDesc c = fr[t.child];
+  const FrontDesc p = fr[c.parent];
+  const int uc = c.m - c.s, sp = p.s, mp = p.m, up = mp - sp, tot = uc * uc;
+  const T* Uc = U + c.offU;
+  const int* r = rel + c.rel;
+  T sv[8], bv[8];
+  T* dst[8];
+  bool on[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int e = (int)threadIdx.x + 256 * q, j = e / uc, i = e - j * uc;
+    on[q] = e < tot && i >= j;
+    sv[q] = (T)0;
+    dst[q] = nullptr;
+    if (on[q]) {
+      sv[q] = Uc[e];
+      const int pi = r[i], pj = r[j];
+      dst[q] = pj < sp ? L + (p.offL + (long long)pj * mp + pi)
+                       : U + (p.offU + (long long)(pj - sp) * up + (pi - sp));
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+    if (on[q]) bv[q] = *dst[q];
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+    if (on[q]) *dst[q] = bv[q] + sv[q];
+}
+
 // One parent per workgroup: heads[b] = {first, count} names the tasks (whole
 // children, in sibling-rank order) added one after the other, so each parent
 // entry receives its children's values in the same order as under the
@@ -1882,7 +1921,13 @@ __global__ __launch_bounds__(256) void k_extend_add_seq(const FrontDesc* __restr
   const EATask h = heads[blockIdx.x];
   for (int q = 0; q < h.j0; q++) {
     if (q > 0) __syncthreads();
-    extend_add_cols(fr, tasks[h.child + q], rel, L, U);
+    const EATask t = tasks[h.child + q];
+    const FrontDesc c = fr[t.child];
+    const int uc = c.m - c.s;
+    if (t.j0 == 0 && t.j1 == uc && uc * uc <= EA_FLAT_MAX)
+      extend_add_flat(fr, t, rel, L, U);
+    else
+      extend_add_cols(fr, t, rel, L, U);
   }
 }
 
