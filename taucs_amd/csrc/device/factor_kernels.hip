@@ -1859,15 +1859,6 @@ __device__ __forceinline__ void extend_add_cols(const FrontDesc* __restrict__ fr
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict__ fr,
-                                                    const EATask* __restrict__ tasks,
-                                                    const int* __restrict__ rel,
-                                                    T* __restrict__ L, T* __restrict__ U)
-{
-  extend_add_cols(fr, tasks[blockIdx.x], rel, L, U);
-}
-
 // A whole child whose u x u block has at most 2048 entries, in one pass of
 // the workgroup: the block's entries as one flat index, eight a thread, the
 // lower ones (i >= j) added into the parent with all loads before the stores
@@ -1905,6 +1896,21 @@ __device__ __forceinline__ void extend_add_flat(const FrontDesc* __restrict__ fr
 #pragma unroll
   for (int q = 0; q < 8; q++)
     if (on[q]) *dst[q] = bv[q] + sv[q];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_extend_add(const FrontDesc* __restrict__ fr,
+                                                    const EATask* __restrict__ tasks,
+                                                    const int* __restrict__ rel,
+                                                    T* __restrict__ L, T* __restrict__ U)
+{
+  const EATask t = tasks[blockIdx.x];
+  const FrontDesc c = fr[t.child];
+  const int uc = c.m - c.s;
+  if (t.j0 == 0 && t.j1 == uc && uc * uc <= EA_FLAT_MAX)
+    extend_add_flat(fr, t, rel, L, U);
+  else
+    extend_add_cols(fr, t, rel, L, U);
 }
 
 // One parent per workgroup: heads[b] = {first, count} names the tasks (whole
