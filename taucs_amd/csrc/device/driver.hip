@@ -443,6 +443,10 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
         if (u <= 0) continue;
         const long long off = ub + P->off_U[sn];
         auto& za = (P->u_start[sn] == P->sn_level[sn] ? zero_at : zero_par_at)[P->u_start[sn]];
+        if (u <= 128) {  // a small block whole, as one flat range (column 0 of a u*u x 1 block)
+          za.push_back({off, (int)(u * u), 0, 1, 0});
+          continue;
+        }
         // the lower triangle, by column ranges of about 64K elements
         long long cnt = 0;
         for (int c = 0, c0 = 0; c < (int)u; c++) {

@@ -16,10 +16,11 @@ __global__ void k_fill(T* __restrict__ p, long long n)
   for (; i < n; i += stride) p[i] = (T)0;
 }
 
-// Zero the update blocks whose life starts at this level, lower triangles
-// only (one task = a column range of at most 64K elements): every reader of
-// an update block (the extend-add, the SYRK epilogue, the whole-front
-// kernels) touches entries (i, j) with i >= j alone.
+// Zero the update blocks whose life starts at this level: the lower
+// triangles of large blocks (one task = a column range of about 64K
+// elements; every reader of an update block, the extend-add, the SYRK
+// epilogue and the whole-front kernels, touches entries (i, j) with i >= j
+// alone), small blocks whole as one flat range.
 template <typename T>
 __global__ void k_fill_ranges(const ZeroTask* __restrict__ tasks, T* __restrict__ U)
 {

@@ -56,6 +56,7 @@ struct SmallPack {  // fronts with m <= width, one per `width`-lane slot of a wa
 };
 
 struct ZeroTask {   // rows c..u-1 of columns c0..c1-1 of the u x u block at U[off] <- 0
+                    // (u = n, c0 = 0, c1 = 1: the n elements from off)
   long long off;
   int u, c0, c1, pad;
 };
