@@ -30,7 +30,7 @@ template <typename T> __global__ void k_front_slot16(const FrontDesc*, const Sma
 template <typename T> __global__ void k_front_slot32(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int);
 template <typename T> __global__ void k_front_w48(const FrontDesc*, const int*, T*, T*, T*, int*, int);
 template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*, const T*);
-template <typename T> __global__ void k_trsm_tile(const GemmTask*, const T*, T*, T*, const T*);
+template <typename T> __global__ void k_panel_solve(const SolveTask*, T*, const T*);
 template <typename T> __global__ void k_gemm_tile128(const GemmTask*, const T*, T*, T*);
 template <typename T> __global__ void k_gemm_short(const GemmTask*, const T*, T*, T*, const T*);
 __global__ void k_gemm_tile_bf16(const GemmTask*, const float*, float*, float*);
@@ -99,7 +99,7 @@ struct tamd_dev {
   int* d_small = nullptr;        // front indices for k_front_small / k_front_wave
   SmallPack* d_packs = nullptr;  // fronts packed 4 (m <= 16) or 2 (m <= 32) to a wave
   PanelTask* d_potrf = nullptr;
-  GemmTask* d_trsm = nullptr;   // panel solves as GEMMs with the block inverses
+  SolveTask* d_trsm = nullptr;  // panel steps: left-looking updates and solves of 64-row tiles
   GemmTask* d_gemm = nullptr;
   EATask* d_ea = nullptr;
   ZeroTask* d_zero = nullptr;
@@ -424,7 +424,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
   std::vector<int> small;
   std::vector<SmallPack> packs;
   std::vector<PanelTask> potrf;
-  std::vector<GemmTask> trsm;
+  std::vector<SolveTask> trsm;
   std::vector<GemmTask> gemm;
   std::vector<EATask> ea;
   std::vector<ZeroTask> zero;
@@ -546,7 +546,7 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     std::vector<GemmTask> rest;
     for (int os = 0; os < maxouter; os++) {
       for (int j = 0; j < W / NB; j++) {
-        int p0 = (int)potrf.size(), t0 = (int)trsm.size(), g0 = (int)gemm.size();
+        int p0 = (int)potrf.size(), t0 = (int)trsm.size();
         for (int k = b0; k < b1; k++) {
           const FrontDesc& d = fr[k];
           if (d.m <= SMALL_M) continue;
@@ -554,21 +554,25 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
           if (k0 >= d.s) continue;
           const int kb = std::min(NB, d.s - k0), pend = std::min(K0 + W, d.s);
           potrf.push_back({k, k0, kb, 0, 0});
+          // one task per 64-row tile below the block (k_panel_solve): its
+          // updates from the outer panel's earlier blocks, its solve, and for
+          // a tile that starts a later diagonal block of the outer panel that
+          // block's update from this step
           for (int r0 = k0 + kb; r0 < d.m; r0 += 64) {
-            GemmTask g;
-            g.a_off = d.offL + r0 + (long long)k0 * d.m;
-            g.b_off = d.offI + (long long)(k0 / 64) * 4096;
-            g.c_off = g.a_off;
-            g.ld = d.m;
-            g.ldc = d.m;
-            g.M = std::min(64, d.m - r0);
-            g.N = kb;
-            g.K = kb;
-            g.flags = 4;
-            trsm.push_back(g);
+            SolveTask st;
+            st.c_off = d.offL + r0 + (long long)k0 * d.m;
+            st.inv_off = d.offI + (long long)(k0 / 64) * 4096;
+            st.xr_off = d.offL + r0 + (long long)K0 * d.m;
+            st.xk_off = d.offL + k0 + (long long)K0 * d.m;
+            st.d_off = d.offL + r0 + (long long)r0 * d.m;
+            st.ld = d.m;
+            st.M = std::min(64, d.m - r0);
+            st.N = kb;
+            st.kpre = k0 - K0;
+            st.Md = r0 < pend ? std::min(64, d.s - r0) : 0;
+            st.pad = 0;
+            trsm.push_back(st);
           }
-          for (int c0 = k0 + kb; c0 < pend; c0 += 64)
-            for (int r0 = c0; r0 < d.m; r0 += 64) gemm.push_back(tile(d, r0, c0, k0, kb, 64));
         }
         Op o;
         if ((int)potrf.size() > p0) {
@@ -578,11 +582,6 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
         }
         if ((int)trsm.size() > t0) {
           o = {TAMD_OP_TRSM, t0, (int)trsm.size() - t0, l};
-          o.nojoin = true;
-          D->ops.push_back(o);
-        }
-        if ((int)gemm.size() > g0) {
-          o = {TAMD_OP_GEMM, g0, (int)gemm.size() - g0, l};
           o.nojoin = true;
           D->ops.push_back(o);
         }
@@ -1006,8 +1005,10 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
     if (op.kind == TAMD_OP_GEMM)
       for (int q = op.start; q < op.start + op.count; q++) f += tile_flops(gemm[q]);
     else if (op.kind == TAMD_OP_TRSM)
-      for (int q = op.start; q < op.start + op.count; q++)
-        f += 2.0 * trsm[q].M * trsm[q].N * trsm[q].K;
+      for (int q = op.start; q < op.start + op.count; q++) {
+        const SolveTask& st = trsm[q];
+        f += 2.0 * st.M * st.N * (st.N + st.kpre) + (double)st.Md * (st.Md + 1) * st.N;
+      }
     else if (op.kind == TAMD_OP_POTRF)
       f = op.count * (2.0 / 3.0) * 64.0 * 64.0 * 64.0;
     else if (op.kind == TAMD_OP_EXTEND)
@@ -1033,7 +1034,8 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
                     + sizeof(int) * (small.size() + colptr.size() + 1)
                     + sizeof(SmallPack) * packs.size()
                     + sizeof(PanelTask) * (potrf.size() + svt.size())
-                    + sizeof(GemmTask) * (trsm.size() + gemm.size()) + sizeof(EATask) * ea.size()
+                    + sizeof(SolveTask) * trsm.size() + sizeof(GemmTask) * gemm.size()
+                    + sizeof(EATask) * ea.size()
                     + sizeof(ZeroTask) * zero.size()
                     + sizeof(double) * (size_t)nnz_tot;
   if (dry) return 0;
@@ -1319,8 +1321,8 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
                (const PanelTask*)(D->d_potrf + op.start), L, Iv, D->d_info);
         break;
       case TAMD_OP_TRSM:
-        EMIT(lane, join, k_trsm_tile<T>, dim3(op.count), dim3(256),
-             (const GemmTask*)(D->d_trsm + op.start), (const T*)L, L, U, (const T*)Iv);
+        EMIT(lane, join, k_panel_solve<T>, dim3(op.count), dim3(256),
+             (const SolveTask*)(D->d_trsm + op.start), L, (const T*)Iv);
         break;
       case TAMD_OP_GEMM:
         if (bf16upd)

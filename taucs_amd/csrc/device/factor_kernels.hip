@@ -1618,15 +1618,8 @@ __global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ 
 {
   gemm_tile<T, 1>(tasks, Lsrc, L, U, Inv);
 }
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_tile(const GemmTask* __restrict__ tasks,
-                                                   const T* Lsrc, T* L, T* U,
-                                                   const T* __restrict__ Inv)
-{
-  __builtin_amdgcn_s_setprio(2);
-  gemm_tile_short<T>(tasks, Lsrc, L, U, Inv);
-}
-// Panel-internal updates (K <= 64), on the panel chain's critical path.
+// Updates with K <= 64 (the trailing update of a front's last outer panel
+// when it is at most 64 columns wide).
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_short(const GemmTask* __restrict__ tasks,
                                                     const T* Lsrc, T* L, T* U,
@@ -1634,6 +1627,188 @@ __global__ __launch_bounds__(256) void k_gemm_short(const GemmTask* __restrict__
 {
   __builtin_amdgcn_s_setprio(2);
   gemm_tile_short<T>(tasks, Lsrc, L, U, Inv);
+}
+
+// One 64-row tile of a panel step, left-looking inside the outer panel (one
+// launch per step on the panel chain, instead of a solve launch and then an
+// update launch): with X the panel's columns already solved,
+//   C  <- C - Xr Xk^T   the tile's updates from the outer panel's earlier
+//                       blocks (Xr: the tile's rows, Xk: the block's rows), in
+//                       one accumulation over their kpre columns;
+//   X  <- C inv(L_kk)^T the solve by the block inverse (C staged through LDS as
+//                       the A operand), stored in place;
+//   Cd <- Cd - X X^T    for a tile whose first row starts a later diagonal
+//                       block of the outer panel: that block's update from
+//                       this step (lower part), so that its POTRF finds it
+//                       complete.
+// The other tiles of the later blocks receive their updates when their own
+// step comes.  MFMA layout and 16-deep K chunks as in gemm_tile_short: wave w
+// holds the 32 x 32 sub-tile (wr, wc), element r of acc[a][b] at row
+// wr + 16a + c16 and column wc + 16b + drow(lane, r).
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_solve(const SolveTask* __restrict__ tasks, T* L,
+                                                     const T* __restrict__ Inv)
+{
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ __align__(16) T As[2][64][KP];
+  __shared__ __align__(16) T Bs[2][64][KP];
+  __builtin_amdgcn_s_setprio(2);
+  const SolveTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int c16 = lane & 15, g = lane >> 4;
+  const int li = tid & 63, lk = tid >> 6;  // loader: row li, k = 16 ch + lk + 4q
+  const long long ld = t.ld;
+  // issued first: the tile (accumulator layout) and the inverse (loader layout)
+  acc_t C[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = wr + 16 * a + c16, j = wc + 16 * b + Mfma<T>::drow(lane, r);
+        C[a][b][r] = (i < t.M && j < t.N) ? L[t.c_off + i + j * ld] : (T)0;
+      }
+  T rv[4][TK / 4];
+#pragma unroll
+  for (int ch = 0; ch < 4; ch++)
+#pragma unroll
+    for (int q = 0; q < TK / 4; q++) {
+      const int k = ch * TK + lk + 4 * q;
+      rv[ch][q] = (li < t.N && k < t.N) ? Inv[t.inv_off + li * 64 + k] : (T)0;
+    }
+  // one 64 x 64 x 64 product of LDS-staged halves into acc (fa rows wr..,
+  // fb rows wc.. of Bsrc, chunks c of the half)
+  auto mma_half = [&](acc_t (&acc)[2][2], T (*Asrc)[64][KP], T (*Bsrc)[64][KP]) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      T fa0[TK / 4], fa1[TK / 4], fb0[TK / 4], fb1[TK / 4];
+#pragma unroll
+      for (int s2 = 0; s2 < TK / 4; s2++) {
+        fa0[s2] = Asrc[c][wr + c16][g * (TK / 4) + s2];
+        fa1[s2] = Asrc[c][wr + 16 + c16][g * (TK / 4) + s2];
+        fb0[s2] = Bsrc[c][wc + c16][g * (TK / 4) + s2];
+        fb1[s2] = Bsrc[c][wc + 16 + c16][g * (TK / 4) + s2];
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < TK / 4; s2++) {
+        acc[0][0] = Mfma<T>::step(fb0[s2], fa0[s2], acc[0][0]);
+        acc[0][1] = Mfma<T>::step(fb1[s2], fa0[s2], acc[0][1]);
+        acc[1][0] = Mfma<T>::step(fb0[s2], fa1[s2], acc[1][0]);
+        acc[1][1] = Mfma<T>::step(fb1[s2], fa1[s2], acc[1][1]);
+      }
+    }
+  };
+  // a matrix in the accumulator layout into As, half h (its columns 32h..32h+31
+  // are held by the waves with wc == 32h), as the A operand's chunks
+  auto stage_acc = [&](const acc_t (&M)[2][2], int h) {
+    if (wc == 32 * h) {
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) As[b][wr + 16 * a + c16][Mfma<T>::drow(lane, r)] = M[a][b][r];
+    }
+  };
+  acc_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
+  // C -= Xr Xk^T over the earlier blocks, 64 columns at a time, the next
+  // block's loads issued before the current block's second half
+  if (t.kpre > 0) {
+    T ra[4][TK / 4], rb[4][TK / 4];
+    auto fetch = [&](int k0) {
+#pragma unroll
+      for (int ch = 0; ch < 4; ch++)
+#pragma unroll
+        for (int q = 0; q < TK / 4; q++) {
+          const long long k = k0 + ch * TK + lk + 4 * q;
+          ra[ch][q] = li < t.M ? L[t.xr_off + li + k * ld] : (T)0;
+          rb[ch][q] = li < t.N ? L[t.xk_off + li + k * ld] : (T)0;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < t.kpre; k0 += 64) {
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        __syncthreads();  // the previous half's fragments are read
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+          for (int q = 0; q < TK / 4; q++) {
+            As[c][li][lk + 4 * q] = ra[2 * h + c][q];
+            Bs[c][li][lk + 4 * q] = rb[2 * h + c][q];
+          }
+        if (h == 1 && k0 + 64 < t.kpre) fetch(k0 + 64);
+        __syncthreads();
+        mma_half(acc, As, Bs);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) C[a][b][r] -= acc[a][b][r];
+  // X = C inv^T
+  acc_t X[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) X[a][b][r] = (T)0;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    __syncthreads();
+    stage_acc(C, h);
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int q = 0; q < TK / 4; q++) Bs[c][li][lk + 4 * q] = rv[2 * h + c][q];
+    __syncthreads();
+    mma_half(X, As, Bs);
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = wr + 16 * a + c16, j = wc + 16 * b + Mfma<T>::drow(lane, r);
+        if (i < t.M && j < t.N) L[t.c_off + i + j * ld] = X[a][b][r];
+      }
+  if (t.Md == 0) return;
+  // Cd -= X X^T (both operands X, staged once into As)
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    __syncthreads();
+    stage_acc(X, h);
+    __syncthreads();
+    mma_half(acc, As, As);
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = wr + 16 * a + c16, j = wc + 16 * b + Mfma<T>::drow(lane, r);
+        if (i < t.Md && j < t.Md && i >= j) L[t.d_off + i + j * ld] -= acc[a][b][r];
+      }
 }
 
 // 128 x 128 tile for the large trailing updates and SYRKs: 8 waves (2 x 4),
@@ -1953,7 +2128,7 @@ __global__ __launch_bounds__(256) void k_extend_add_seq(const FrontDesc* __restr
   template __global__ void k_front_slot32<T>(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int); \
   template __global__ void k_front_w48<T>(const FrontDesc*, const int*, T*, T*, T*, int*, int);          \
   template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
-  template __global__ void k_trsm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
+  template __global__ void k_panel_solve<T>(const SolveTask*, T*, const T*);               \
   template __global__ void k_gemm_tile128<T>(const GemmTask*, const T*, T*, T*);          \
   template __global__ void k_gemm_short<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*); \

@@ -43,6 +43,18 @@ struct GemmTask {   // C[i,j] -= sum_k A[i,k] * B[j,k] on one 64x64 tile
                     //    (the panel triangular solve X <- X L_kk^{-T}, in place)
 };
 
+struct SolveTask {  // one 64-row tile of a panel step (k_panel_solve)
+  long long c_off;   // the tile: C(i, j) = L[c_off + i + j*ld], i < M, j < N (the step's block)
+  long long inv_off; // the block inverse, row-major: Inv[inv_off + j*64 + k]
+  long long xr_off;  // the tile's rows in the outer panel's earlier blocks: L[xr_off + i + k*ld]
+  long long xk_off;  // the block's own rows in those blocks: L[xk_off + j + k*ld]
+  long long d_off;   // the diagonal block starting at the tile's first row: L[d_off + i + j*ld]
+  int ld, M, N;
+  int kpre;          // columns of the earlier blocks (k < kpre; 0 at an outer panel's first step)
+  int Md;            // order of that diagonal block when it lies in the outer panel, else 0
+  int pad;
+};
+
 struct PanelTask {  // one diagonal block, or one row block of the panel
   int front;
   int k0, kb;       // columns k0 .. k0+kb-1 of the panel
