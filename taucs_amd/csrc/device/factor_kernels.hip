@@ -1659,7 +1659,7 @@ __global__ __launch_bounds__(256) void k_panel_solve(const SolveTask* __restrict
   const int c16 = lane & 15, g = lane >> 4;
   const int li = tid & 63, lk = tid >> 6;  // loader: row li, k = 16 ch + lk + 4q
   const long long ld = t.ld;
-  // issued first: the tile (accumulator layout) and the inverse (loader layout)
+  // issued first: the tile (accumulator layout)
   acc_t C[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -1670,14 +1670,6 @@ __global__ __launch_bounds__(256) void k_panel_solve(const SolveTask* __restrict
         const int i = wr + 16 * a + c16, j = wc + 16 * b + Mfma<T>::drow(lane, r);
         C[a][b][r] = (i < t.M && j < t.N) ? L[t.c_off + i + j * ld] : (T)0;
       }
-  T rv[4][TK / 4];
-#pragma unroll
-  for (int ch = 0; ch < 4; ch++)
-#pragma unroll
-    for (int q = 0; q < TK / 4; q++) {
-      const int k = ch * TK + lk + 4 * q;
-      rv[ch][q] = (li < t.N && k < t.N) ? Inv[t.inv_off + li * 64 + k] : (T)0;
-    }
   // one 64 x 64 x 64 product of LDS-staged halves into acc (fa rows wr..,
   // fb rows wc.. of Bsrc, chunks c of the half)
   auto mma_half = [&](acc_t (&acc)[2][2], T (*Asrc)[64][KP], T (*Bsrc)[64][KP]) {
@@ -1719,10 +1711,22 @@ __global__ __launch_bounds__(256) void k_panel_solve(const SolveTask* __restrict
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
+  // the inverse in the loader layout: loaded into rb once the last earlier
+  // block is staged (at once without earlier blocks)
+  T ra[4][TK / 4], rb[4][TK / 4];
+  auto fetch_inv = [&]() {
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++)
+#pragma unroll
+      for (int q = 0; q < TK / 4; q++) {
+        const int k = ch * TK + lk + 4 * q;
+        rb[ch][q] = (li < t.N && k < t.N) ? Inv[t.inv_off + li * 64 + k] : (T)0;
+      }
+  };
   // C -= Xr Xk^T over the earlier blocks, 64 columns at a time, the next
   // block's loads issued before the current block's second half
+  if (t.kpre == 0) fetch_inv();
   if (t.kpre > 0) {
-    T ra[4][TK / 4], rb[4][TK / 4];
     auto fetch = [&](int k0) {
 #pragma unroll
       for (int ch = 0; ch < 4; ch++)
@@ -1745,7 +1749,10 @@ __global__ __launch_bounds__(256) void k_panel_solve(const SolveTask* __restrict
             As[c][li][lk + 4 * q] = ra[2 * h + c][q];
             Bs[c][li][lk + 4 * q] = rb[2 * h + c][q];
           }
-        if (h == 1 && k0 + 64 < t.kpre) fetch(k0 + 64);
+        if (h == 1) {
+          if (k0 + 64 < t.kpre) fetch(k0 + 64);
+          else fetch_inv();
+        }
         __syncthreads();
         mma_half(acc, As, Bs);
       }
@@ -1772,7 +1779,7 @@ __global__ __launch_bounds__(256) void k_panel_solve(const SolveTask* __restrict
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
-      for (int q = 0; q < TK / 4; q++) Bs[c][li][lk + 4 * q] = rv[2 * h + c][q];
+      for (int q = 0; q < TK / 4; q++) Bs[c][li][lk + 4 * q] = rb[2 * h + c][q];
     __syncthreads();
     mma_half(X, As, Bs);
   }
