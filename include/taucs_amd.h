@@ -179,10 +179,12 @@ void  taucs_amd_set_graphs(int on);
 /* Panel lookahead: the trailing update of a panel step runs on a second stream
    while the next panel is factored (default on; results are identical). */
 void  taucs_amd_set_lookahead(int on);
-/* Diagonal-block POTRF kernel: 2 = four waves, 16-column panels with MFMA
-   updates (default); 1 = one wave, register broadcasts; 0 = 256 threads
-   through LDS.  2 and 1 give bitwise equal factors.  Applies to numeric
-   factorizations started afterwards. */
+/* Diagonal-block POTRF kernel for the block that starts each outer panel:
+   2 = four waves, 16-column panels with MFMA updates (default); 1 = one
+   wave, register broadcasts; 0 = 256 threads through LDS.  2 and 1 give
+   bitwise equal factors.  The other diagonal blocks are factored by the
+   four-wave code at the end of the previous panel step's launch.  Applies
+   to numeric factorizations started afterwards. */
 void  taucs_amd_set_potrf_kernel(int kind);
 /* Fronts with at most 16 (32) rows are factored four (two) to a wavefront,
    and fronts with at most 48 rows with 48-wide staging (default on); the

@@ -30,7 +30,7 @@ template <typename T> __global__ void k_front_slot16(const FrontDesc*, const Sma
 template <typename T> __global__ void k_front_slot32(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int);
 template <typename T> __global__ void k_front_w48(const FrontDesc*, const int*, T*, T*, T*, int*, int);
 template <typename T> __global__ void k_gemm_tile(const GemmTask*, const T*, T*, T*, const T*);
-template <typename T> __global__ void k_panel_solve(const SolveTask*, T*, const T*);
+template <typename T> __global__ void k_panel_solve(const SolveTask*, T*, T*, const FrontDesc*, int*, int);
 template <typename T> __global__ void k_gemm_tile128(const GemmTask*, const T*, T*, T*);
 template <typename T> __global__ void k_gemm_short(const GemmTask*, const T*, T*, T*, const T*);
 __global__ void k_gemm_tile_bf16(const GemmTask*, const float*, float*, float*);
@@ -176,9 +176,11 @@ static int g_tile_big = 64;
 extern "C" void taucs_amd_set_gemm_tile(int edge) { g_tile_big = edge == 128 ? 128 : 64; }
 static int g_lookahead = 1;
 extern "C" void taucs_amd_set_lookahead(int on) { g_lookahead = on; }
-// Diagonal-block kernel: 2 = four waves, 16-column panels with MFMA updates
-// (default); 1 = one wave with register broadcasts; 0 = 256 threads through
-// LDS (kept for comparison).  2 and 1 give bitwise equal factors.
+// Diagonal-block kernel of the block that starts each outer panel: 2 = four
+// waves, 16-column panels with MFMA updates (default); 1 = one wave with
+// register broadcasts; 0 = 256 threads through LDS (kept for comparison).  2
+// and 1 give bitwise equal factors.  The later blocks of an outer panel are
+// factored by the four-wave code inside the previous step's k_panel_solve.
 static int g_potrf_wave = 2;
 extern "C" void taucs_amd_set_potrf_kernel(int kind) { g_potrf_wave = kind >= 0 && kind <= 2 ? kind : 2; }
 // Small fronts several to a wavefront: m <= 16 four per wave (k_front_slot16),
@@ -553,7 +555,9 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
           const int K0 = os * W, k0 = K0 + j * NB;
           if (k0 >= d.s) continue;
           const int kb = std::min(NB, d.s - k0), pend = std::min(K0 + W, d.s);
-          potrf.push_back({k, k0, kb, 0, 0});
+          // an outer panel's first block waits for the trailing update; the
+          // later ones are factored at the end of the previous step's launch
+          if (k0 == K0) potrf.push_back({k, k0, kb, 0, 0});
           // one task per 64-row tile below the block (k_panel_solve): its
           // updates from the outer panel's earlier blocks, its solve, and for
           // a tile that starts a later diagonal block of the outer panel that
@@ -570,7 +574,8 @@ static int dev_build(tamd_dev* D, const std::vector<SysView>& sys, bool dry = fa
             st.N = kb;
             st.kpre = k0 - K0;
             st.Md = r0 < pend ? std::min(64, d.s - r0) : 0;
-            st.pad = 0;
+            st.pf_front = r0 == k0 + kb && r0 < pend ? k : -1;
+            st.pf_k0 = r0;
             trsm.push_back(st);
           }
         }
@@ -1322,7 +1327,8 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
         break;
       case TAMD_OP_TRSM:
         EMIT(lane, join, k_panel_solve<T>, dim3(op.count), dim3(256),
-             (const SolveTask*)(D->d_trsm + op.start), L, (const T*)Iv);
+             (const SolveTask*)(D->d_trsm + op.start), L, Iv, (const FrontDesc*)D->d_fronts,
+             D->d_info, g_fp64_fallback);
         break;
       case TAMD_OP_GEMM:
         if (bf16upd)

@@ -555,24 +555,28 @@ __device__ __forceinline__ void pm_block_update(T* A, int Q, int R, int p, int l
   }
 }
 
+// LDS of potrf_mw_block, in bytes: the block A (64 x PM_LDA), rdiag, dg, the
+// four waves' 16 x 17 staging tiles Ts, and one flag
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf_mw(const FrontDesc* __restrict__ fr,
-                                                  const PanelTask* __restrict__ tasks,
-                                                  T* __restrict__ L, T* __restrict__ Inv,
-                                                  int* __restrict__ info, int fallback)
+constexpr int potrf_mw_lds() { return (int)sizeof(T) * (64 * PM_LDA + 64 + 64 + 4 * 16 * 17) + 16; }
+
+template <typename T>
+__device__ __forceinline__ void potrf_mw_block(const FrontDesc* __restrict__ fr, const PanelTask t,
+                                               T* __restrict__ L, T* __restrict__ Inv,
+                                               int* __restrict__ info, int fallback,
+                                               unsigned char* smem)
 {
   typedef typename Mfma<T>::acc_t acc_t;
   // L in the lower triangle; its inverse X, transposed, in the upper one
   // (X(r, c) at A[r*PM_LDA + c], r >= c; L's diagonal is kept in rdiag)
   // (FP32: the FP64 fallback's packed triangle, 2080 doubles, takes the
-  // same 16,640 bytes)
-  __shared__ __align__(16) T A[64 * PM_LDA];
-  __shared__ __align__(16) T rdiag[64];
-  __shared__ __align__(16) T dg[64];
-  __shared__ __align__(16) T Ts[4][16 * 17];
-  __shared__ int flagged;
+  // same 16,640 bytes); all in the caller's LDS buffer of potrf_mw_lds<T>()
+  T* const A = reinterpret_cast<T*>(smem);
+  T* const rdiag = A + 64 * PM_LDA;
+  T* const dg = rdiag + 64;
+  T (*const Ts)[16 * 17] = reinterpret_cast<T (*)[16 * 17]>(dg + 64);
+  int& flagged = *reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(dg + 64 + 4 * 16 * 17));
   __builtin_amdgcn_s_setprio(3);  // the panel chain is the critical path
-  const PanelTask t = tasks[blockIdx.x];
   const FrontDesc d = fr[t.front];
   const long long m = d.m;
   const int kb = t.kb, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -736,6 +740,16 @@ __global__ __launch_bounds__(256) void k_potrf_mw(const FrontDesc* __restrict__ 
     const int idx = tid + 256 * q, i = idx >> 6, j = idx & 63;
     iv[idx] = (i < kb && j < kb && i >= j) ? A[i * PM_LDA + j] : (T)0;
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_mw(const FrontDesc* __restrict__ fr,
+                                                  const PanelTask* __restrict__ tasks,
+                                                  T* __restrict__ L, T* __restrict__ Inv,
+                                                  int* __restrict__ info, int fallback)
+{
+  __shared__ __align__(16) unsigned char smem[potrf_mw_lds<T>()];
+  potrf_mw_block<T>(fr, tasks[blockIdx.x], L, Inv, info, fallback, smem);
 }
 
 // Whole front with m <= 64 in one wavefront: lane i holds row i of the
@@ -1642,16 +1656,23 @@ __global__ __launch_bounds__(256) void k_gemm_short(const GemmTask* __restrict__
 //                       this step (lower part), so that its POTRF finds it
 //                       complete.
 // The other tiles of the later blocks receive their updates when their own
-// step comes.  MFMA layout and 16-deep K chunks as in gemm_tile_short: wave w
-// holds the 32 x 32 sub-tile (wr, wc), element r of acc[a][b] at row
-// wr + 16a + c16 and column wc + 16b + drow(lane, r).
+// step comes.  The workgroup of the tile that starts the next step's block
+// then factors that block (potrf_mw_block, as k_potrf_mw would in a launch of
+// its own), so an outer panel's steps after the first need one launch each.
+// MFMA layout and 16-deep K chunks as in gemm_tile_short: wave w holds the
+// 32 x 32 sub-tile (wr, wc), element r of acc[a][b] at row wr + 16a + c16 and
+// column wc + 16b + drow(lane, r).
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_solve(const SolveTask* __restrict__ tasks, T* L,
-                                                     const T* __restrict__ Inv)
+                                                     T* Inv, const FrontDesc* __restrict__ fr,
+                                                     int* __restrict__ info, int fallback)
 {
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ __align__(16) T As[2][64][KP];
-  __shared__ __align__(16) T Bs[2][64][KP];
+  // one LDS buffer: the staging halves As, Bs, then the next block's POTRF
+  constexpr int STAGE = 2 * 2 * 64 * KP * (int)sizeof(T);
+  __shared__ __align__(16) unsigned char smem[STAGE > potrf_mw_lds<T>() ? STAGE : potrf_mw_lds<T>()];
+  T (*const As)[64][KP] = reinterpret_cast<T (*)[64][KP]>(smem);
+  T (*const Bs)[64][KP] = As + 2;
   __builtin_amdgcn_s_setprio(2);
   const SolveTask t = tasks[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1816,6 +1837,9 @@ __global__ __launch_bounds__(256) void k_panel_solve(const SolveTask* __restrict
         const int i = wr + 16 * a + c16, j = wc + 16 * b + Mfma<T>::drow(lane, r);
         if (i < t.Md && j < t.Md && i >= j) L[t.d_off + i + j * ld] -= acc[a][b][r];
       }
+  if (t.pf_front < 0) return;
+  __syncthreads();  // the block's update is stored, and As read, before the POTRF
+  potrf_mw_block<T>(fr, PanelTask{t.pf_front, t.pf_k0, t.Md, 0, 0}, L, Inv, info, fallback, smem);
 }
 
 // 128 x 128 tile for the large trailing updates and SYRKs: 8 waves (2 x 4),
@@ -2135,7 +2159,7 @@ __global__ __launch_bounds__(256) void k_extend_add_seq(const FrontDesc* __restr
   template __global__ void k_front_slot32<T>(const FrontDesc*, const SmallPack*, T*, T*, T*, int*, int); \
   template __global__ void k_front_w48<T>(const FrontDesc*, const int*, T*, T*, T*, int*, int);          \
   template __global__ void k_gemm_tile<T>(const GemmTask*, const T*, T*, T*, const T*);  \
-  template __global__ void k_panel_solve<T>(const SolveTask*, T*, const T*);               \
+  template __global__ void k_panel_solve<T>(const SolveTask*, T*, T*, const FrontDesc*, int*, int); \
   template __global__ void k_gemm_tile128<T>(const GemmTask*, const T*, T*, T*);          \
   template __global__ void k_gemm_short<T>(const GemmTask*, const T*, T*, T*, const T*);  \
   template __global__ void k_extend_add<T>(const FrontDesc*, const EATask*, const int*, T*, T*); \

@@ -52,7 +52,8 @@ struct SolveTask {  // one 64-row tile of a panel step (k_panel_solve)
   int ld, M, N;
   int kpre;          // columns of the earlier blocks (k < kpre; 0 at an outer panel's first step)
   int Md;            // order of that diagonal block when it lies in the outer panel, else 0
-  int pad;
+  int pf_front;      // the next step's diagonal block, factored by this tile's workgroup once
+  int pf_k0;         // it is updated: front, first column (pf_front < 0: none)
 };
 
 struct PanelTask {  // one diagonal block, or one row block of the panel
