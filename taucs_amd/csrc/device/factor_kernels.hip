@@ -852,6 +852,41 @@ __device__ __forceinline__ void guw_store(T* gu, int li, int s, int c, int u, T 
   gu[(li - s) + (long long)(c - s) * u] = v;
 }
 
+// Row li of a small front into registers, columns 0..W-1: the panel columns
+// from L, the update block's lower triangle from U, and 1 on the diagonal, 0
+// elsewhere, for entries right of the diagonal and for rows outside a front
+// (valid false).  Every load is issued unconditionally (an entry outside the
+// row reads the front's first entry and drops it), so all W are in flight
+// together rather than a few per trip of a loop bounded by li.
+template <typename T, int W>
+__device__ __forceinline__ void load_front_row(T (&r)[W], const T* gp, const T* gu, int li, int s,
+                                               int m, int u, bool valid)
+{
+#pragma unroll
+  for (int c = 0; c < W; c++) {
+    const bool ok = valid && c <= li;
+    const T* q = c < s ? gp + li + (long long)c * m : gu + (li - s) + (long long)(c - s) * u;
+    r[c] = *(ok ? q : gp);
+  }
+#pragma unroll
+  for (int c = 0; c < W; c++)
+    if (!(valid && c <= li)) r[c] = li == c ? (T)1 : (T)0;
+}
+
+// The factored row back: L in the panel columns, the Schur complement in the
+// update block's lower triangle (columns 0..li of a valid row).
+template <typename T, int W>
+__device__ __forceinline__ void store_front_row(const T (&r)[W], T* gp, T* gu, int li, int s,
+                                                int m, int u, bool valid)
+{
+#pragma unroll
+  for (int c = 0; c < W; c++)
+    if (valid && c <= li) {
+      if (c < s) gp[li + (long long)c * m] = r[c];
+      else gu[(li - s) + (long long)(c - s) * u] = r[c];
+    }
+}
+
 // Fronts with m <= 16, four to a wavefront, lane = row within its slot of 16
 // lanes and each lane holding only its slot's 16 columns (a fifth of the
 // registers of the 64-wide kernels, so several waves share a SIMD).  Every
@@ -888,21 +923,8 @@ __global__ __launch_bounds__(64) void k_front_slot16(const FrontDesc* __restrict
   const int u = m - s;
   T* const gp = L + offL;
   T* const gu = U + offU;
-  // the lane's row of its front, staged through LDS (one load per entry at
-  // run-time column offsets; the registers take it at fixed indices)
-  T* row = Ls + lane * 17;
-#pragma unroll
-  for (int c = 0; c < 16; c++) row[c] = li == c ? (T)1 : (T)0;
-  if (fi >= 0 && li < m) {
-    for (int c = 0; c <= li; c++) {
-      if (c < s) row[c] = gp[li + (long long)c * m];
-      else row[c] = gu[(li - s) + (long long)(c - s) * u];
-    }
-  }
-  wave_lds_fence();
   T r[16];
-#pragma unroll
-  for (int c = 0; c < 16; c++) r[c] = row[c];
+  load_front_row<T, 16>(r, gp, gu, li, s, m, u, fi >= 0 && li < m);
   if constexpr (sizeof(T) == 4) {
     T v = (T)1;
 #pragma unroll
@@ -966,18 +988,7 @@ __global__ __launch_bounds__(64) void k_front_slot16(const FrontDesc* __restrict
       if (lane == 0) atomicAdd(info + 1, 1);
     }
   }
-  // the factored row back through LDS: L in the panel columns, the Schur
-  // complement in the update block
-  wave_lds_fence();
-#pragma unroll
-  for (int c = 0; c < 16; c++) row[c] = r[c];
-  wave_lds_fence();
-  if (fi >= 0 && li < m) {
-    for (int c = 0; c <= li; c++) {
-      if (c < s) gp[li + (long long)c * m] = row[c];
-      else guw_store(gu, li, s, c, u, row[c]);
-    }
-  }
+  store_front_row<T, 16>(r, gp, gu, li, s, m, u, fi >= 0 && li < m);
   // inverse of each front's leading s x s block (its other rows: identity)
   if (li >= s) {
 #pragma unroll
@@ -1055,21 +1066,8 @@ __global__ __launch_bounds__(64) void k_front_slot32(const FrontDesc* __restrict
   const int u = m - s;
   T* const gp = L + offL;
   T* const gu = U + offU;
-  // the lane's row of its front, staged through LDS (one load per entry at
-  // run-time column offsets; the registers take it at fixed indices)
-  T* row = Ls + lane * SL;
-#pragma unroll
-  for (int c = 0; c < 32; c++) row[c] = li == c ? (T)1 : (T)0;
-  if (fi >= 0 && li < m) {
-    for (int c = 0; c <= li; c++) {
-      if (c < s) row[c] = gp[li + (long long)c * m];
-      else row[c] = gu[(li - s) + (long long)(c - s) * u];
-    }
-  }
-  wave_lds_fence();
   T r[32];
-#pragma unroll
-  for (int c = 0; c < 32; c++) r[c] = row[c];
+  load_front_row<T, 32>(r, gp, gu, li, s, m, u, fi >= 0 && li < m);
   if constexpr (sizeof(T) == 4) {
     T v = (T)1;
 #pragma unroll
@@ -1140,18 +1138,7 @@ __global__ __launch_bounds__(64) void k_front_slot32(const FrontDesc* __restrict
       if (lane == 0) atomicAdd(info + 1, 1);
     }
   }
-  // the factored row back through LDS: L in the panel columns, the Schur
-  // complement in the update block
-  wave_lds_fence();
-#pragma unroll
-  for (int c = 0; c < 32; c++) row[c] = r[c];
-  wave_lds_fence();
-  if (fi >= 0 && li < m) {
-    for (int c = 0; c <= li; c++) {
-      if (c < s) gp[li + (long long)c * m] = row[c];
-      else guw_store(gu, li, s, c, u, row[c]);
-    }
-  }
+  store_front_row<T, 32>(r, gp, gu, li, s, m, u, fi >= 0 && li < m);
   // inverse of each front's leading s x s block (its other rows: identity)
   if (li >= s) {
 #pragma unroll
@@ -1253,21 +1240,8 @@ __global__ __launch_bounds__(64) void k_front_w48(const FrontDesc* __restrict__ 
   const bool act = lane < 48;
   T* const gp = L + d.offL;
   T* const gu = U + d.offU;
-  T* row = Ls + (act ? lane : 0) * SL;
-  if (act) {
-#pragma unroll
-    for (int c = 0; c < 48; c++) row[c] = lane == c ? (T)1 : (T)0;
-    if (lane < m) {
-      for (int c = 0; c <= lane; c++) {
-        if (c < s) row[c] = gp[lane + (long long)c * m];
-        else row[c] = gu[(lane - s) + (long long)(c - s) * u];
-      }
-    }
-  }
-  wave_lds_fence();
-  T r[48];
-#pragma unroll
-  for (int c = 0; c < 48; c++) r[c] = act ? row[c] : (T)0;
+  T r[48];  // lanes 48-63: rows of zeros
+  load_front_row<T, 48>(r, gp, gu, lane, s, m, u, act && lane < m);
   if constexpr (sizeof(T) == 4) {
     T v = (T)1;
 #pragma unroll
@@ -1331,20 +1305,7 @@ __global__ __launch_bounds__(64) void k_front_w48(const FrontDesc* __restrict__ 
       if (lane == 0) atomicAdd(info + 1, 1);
     }
   }
-  // the factored row back through LDS: L in the panel columns, the Schur
-  // complement in the update block
-  wave_lds_fence();
-  if (act) {
-#pragma unroll
-    for (int c = 0; c < 48; c++) row[c] = r[c];
-  }
-  wave_lds_fence();
-  if (act && lane < m) {
-    for (int c = 0; c <= lane; c++) {
-      if (c < s) gp[lane + (long long)c * m] = row[c];
-      else guw_store(gu, lane, s, c, u, row[c]);
-    }
-  }
+  store_front_row<T, 48>(r, gp, gu, lane, s, m, u, act && lane < m);
   // inverse of the leading s x s block (its other rows: identity)
   if (lane >= s) {
 #pragma unroll
