@@ -581,14 +581,22 @@ __device__ __forceinline__ void potrf_mw_block(const FrontDesc* __restrict__ fr,
   const long long m = d.m;
   const int kb = t.kb, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   T* const gp = L + d.offL + t.k0 + (long long)t.k0 * m;
-  // the lower triangle (identity padding beyond kb, zeros above)
+  // the lower triangle (identity padding beyond kb, zeros above), the 16
+  // loads of a thread issued unconditionally and in flight together (an
+  // entry outside the triangle reads the block's first entry and drops it)
+  T v[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) {
     const int c = wave + 4 * q;
-    T v = lane == c ? (T)1 : (T)0;
-    if (lane < kb && c < kb && lane >= c) v = gp[lane + c * m];
-    A[c * PM_LDA + lane] = v;
-    if (lane == c) dg[c] = v;
+    const bool ok = lane < kb && c < kb && lane >= c;
+    v[q] = *(ok ? gp + lane + c * m : gp);
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int c = wave + 4 * q;
+    if (!(lane < kb && c < kb && lane >= c)) v[q] = lane == c ? (T)1 : (T)0;
+    A[c * PM_LDA + lane] = v[q];
+    if (lane == c) dg[c] = v[q];
   }
   if (tid == 0) flagged = 0;
   __syncthreads();
@@ -775,16 +783,21 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
   const int s = d.s, m = d.m, u = m - s, lane = threadIdx.x;
   T* gp = L + d.offL;
   T* gu = U + d.offU;
+  // the panel columns whole, the update block's lower triangle (the rest is
+  // never written), 1 on the diagonal and 0 elsewhere; every load issued
+  // unconditionally (an entry outside the front reads its first entry and
+  // drops it), so that all 64 are in flight together
   T r[64];
 #pragma unroll
   for (int c = 0; c < 64; c++) {
-    T v = lane == c ? (T)1 : (T)0;
-    if (c < s) {
-      if (lane < m) v = gp[lane + c * m];
-    } else if (c < m) {  // the update block's lower triangle (the rest is never written)
-      if (lane >= c && lane < m) v = gu[(lane - s) + (c - s) * u];
-    }
-    r[c] = v;
+    const bool ok = c < s ? lane < m : (c < m && lane >= c && lane < m);
+    const T* q = c < s ? gp + lane + c * m : gu + (lane - s) + (c - s) * u;
+    r[c] = *(ok ? q : gp);
+  }
+#pragma unroll
+  for (int c = 0; c < 64; c++) {
+    const bool ok = c < s ? lane < m : (c < m && lane >= c && lane < m);
+    if (!ok) r[c] = lane == c ? (T)1 : (T)0;
   }
   if constexpr (sizeof(T) == 4) {
     T v = (T)1;
@@ -816,10 +829,10 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
       if (lane == 0) atomicAdd(info + 1, 1);
     }
   }
+  // global stores (not flat: the inverse's LDS waits below would then wait
+  // for them too)
   T* gw = gp;
-  asm volatile("" : "+s"(gw));
   T* guw = gu;
-  asm volatile("" : "+s"(guw));
   int ln = lane;
   asm volatile("" : "+v"(ln));
 #pragma unroll
@@ -844,12 +857,6 @@ __global__ __launch_bounds__(64) void k_front_wave(const FrontDesc* __restrict__
 #pragma unroll
   for (int i = 0; i < 64; i++) iv[i * 64 + ln] = (ln < s && i < s) ? r[i] : (T)0;
   if (lane == 0 && bad) atomicOr(info, 1);
-}
-
-template <typename T>
-__device__ __forceinline__ void guw_store(T* gu, int li, int s, int c, int u, T v)
-{
-  gu[(li - s) + (long long)(c - s) * u] = v;
 }
 
 // Row li of a small front into registers, columns 0..W-1: the panel columns
@@ -1983,13 +1990,12 @@ __device__ __forceinline__ void extend_add_cols(const FrontDesc* __restrict__ fr
         int ri[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          const int i = i0 + 256 * q;
-          sv[q] = i < uc ? src[i] : (T)0;
-          ri[q] = i < uc ? r[i] : 0;
+          const int i = min(i0 + 256 * q, uc - 1);  // rows past the end: read, dropped
+          sv[q] = src[i];
+          ri[q] = r[i];
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (i0 + 256 * q < uc) bv[q] = base[col0 + ri[q]];
+        for (int q = 0; q < 4; q++) bv[q] = base[col0 + ri[q]];
 #pragma unroll
         for (int q = 0; q < 4; q++)
           if (i0 + 256 * q < uc) base[col0 + ri[q]] = bv[q] + sv[q];
@@ -2007,19 +2013,20 @@ __device__ __forceinline__ void extend_add_cols(const FrontDesc* __restrict__ fr
     // four rows per lane per pass, all loads before the stores: the rows of a
     // column map to distinct parent rows, and child and parent blocks never
     // overlap, so the batch is free of conflicts (the compiler cannot know
-    // that: both may point into the update arena)
+    // that: both may point into the update arena).  The loads are issued
+    // unconditionally (a row past the end reads the column's last row and
+    // drops it), so they are in flight together.
     for (int i0 = j + lane; i0 < uc; i0 += 256) {
       T sv[4], bv[4];
       int ri[4];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int i = i0 + 64 * q;
-        sv[q] = i < uc ? src[i] : (T)0;
-        ri[q] = i < uc ? r[i] : 0;
+        const int i = min(i0 + 64 * q, uc - 1);  // rows past the end: read, dropped
+        sv[q] = src[i];
+        ri[q] = r[i];
       }
 #pragma unroll
-      for (int q = 0; q < 4; q++)
-        if (i0 + 64 * q < uc) bv[q] = base[col0 + ri[q]];
+      for (int q = 0; q < 4; q++) bv[q] = base[col0 + ri[q]];
 #pragma unroll
       for (int q = 0; q < 4; q++)
         if (i0 + 64 * q < uc) base[col0 + ri[q]] = bv[q] + sv[q];
@@ -2042,25 +2049,27 @@ __device__ __forceinline__ void extend_add_flat(const FrontDesc* __restrict__ fr
   const int uc = c.m - c.s, sp = p.s, mp = p.m, up = mp - sp, tot = uc * uc;
   const T* Uc = U + c.offU;
   const int* r = rel + c.rel;
+  // every load issued unconditionally (an entry that is not added reads
+  // entry (0, 0) and drops it), so that they are in flight together
   T sv[8], bv[8];
-  T* dst[8];
+  int ri[8], rj[8];
   bool on[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const int e = (int)threadIdx.x + 256 * q, j = e / uc, i = e - j * uc;
     on[q] = e < tot && i >= j;
-    sv[q] = (T)0;
-    dst[q] = nullptr;
-    if (on[q]) {
-      sv[q] = Uc[e];
-      const int pi = r[i], pj = r[j];
-      dst[q] = pj < sp ? L + (p.offL + (long long)pj * mp + pi)
-                       : U + (p.offU + (long long)(pj - sp) * up + (pi - sp));
-    }
+    sv[q] = Uc[on[q] ? e : 0];
+    ri[q] = r[on[q] ? i : 0];
+    rj[q] = r[on[q] ? j : 0];
   }
+  T* dst[8];
 #pragma unroll
-  for (int q = 0; q < 8; q++)
-    if (on[q]) bv[q] = *dst[q];
+  for (int q = 0; q < 8; q++) {
+    const int pi = ri[q], pj = rj[q];
+    dst[q] = pj < sp ? L + (p.offL + (long long)pj * mp + pi)
+                     : U + (p.offU + (long long)(pj - sp) * up + (pi - sp));
+    bv[q] = *dst[q];
+  }
 #pragma unroll
   for (int q = 0; q < 8; q++)
     if (on[q]) *dst[q] = bv[q] + sv[q];
