@@ -1255,6 +1255,9 @@ struct Emitter {
 // 152 registers and 18.5 KiB).
 template <typename T> constexpr int SIDE_WGS = sizeof(T) == 8 ? 2 : 4;
 #define SIDE_PAD_MAX_TILES_FP64 4096
+#ifndef LAT_TILES_MAX
+#define LAT_TILES_MAX 512
+#endif
 template <typename T>
 static unsigned side_lds_pad()
 {
@@ -1337,7 +1340,11 @@ static int launch_numeric(tamd_dev* D, Emitter& E, bool bf16upd, bool prof)
         else if (op.tile == 128)
           EMIT(lane, join, k_gemm_tile128<T>, dim3(op.count), dim3(512),
                (const GemmTask*)(D->d_gemm + op.start), (const T*)L, L, U);
-        else if (op.shortk && lane == 0)
+        // the latency-shaped tile (the same bits) where each workgroup has a
+        // CU to itself: K <= 64, or a main-chain launch of at most
+        // LAT_TILES_MAX tiles (the next outer panel's columns of the top
+        // fronts: K = W in 4 round trips per 256 instead of 16)
+        else if (lane == 0 && (op.shortk || op.count <= LAT_TILES_MAX))
           EMIT(lane, join, k_gemm_short<T>, dim3(op.count), dim3(256),
                (const GemmTask*)(D->d_gemm + op.start), (const T*)L, L, U, (const T*)Iv);
         else {

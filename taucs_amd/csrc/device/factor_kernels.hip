@@ -1501,11 +1501,15 @@ __device__ __forceinline__ void gemm_tile(const GemmTask* __restrict__ tasks, co
       }
 }
 
-// The same tile for K <= 64 (the panel chain's TRSM and the panel-internal
-// updates), shaped for latency: the four 16-deep chunks are fetched at once,
-// so the tile pays one global round trip instead of four, and are staged
-// through LDS two chunks at a time.  The chunks, their LDS layout and the
-// MFMA order are those of gemm_tile: the results are bitwise equal.
+// The same tile shaped for latency (launches with few tiles, where each
+// workgroup has its CU to itself and gemm_tile's one-chunk prefetch pays a
+// global round trip per 16-deep chunk): K in groups of four chunks, each
+// group's loads issued together (branch-free, out-of-range entries read a
+// clamped in-range one and are dropped), so the tile pays one round trip per
+// 64 of K, and the next group's loads are issued once the current group's
+// last two chunks are in LDS, while they are multiplied.  The chunks, their
+// LDS layout and the MFMA order are those of gemm_tile: the results are
+// bitwise equal.
 template <typename T>
 __device__ __forceinline__ void gemm_tile_short(const GemmTask* __restrict__ tasks, const T* Lsrc,
                                                 T* L, T* U, const T* __restrict__ Inv)
@@ -1528,22 +1532,43 @@ __device__ __forceinline__ void gemm_tile_short(const GemmTask* __restrict__ tas
       for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
 
   const int li = tid & 63, lk = tid >> 6;  // loader: row li, k = 16 ch + lk + 4q
+  const int lam = min(li, t.M - 1), lbn = min(li, t.N - 1);
   T ra[4][TK / 4], rb[4][TK / 4];
+  // (the empty asm gives every loaded value an unconditional use, so the
+  // compiler cannot sink a load into the select below and wait for it there)
+  auto fetch4 = [&](int k0) {
 #pragma unroll
-  for (int ch = 0; ch < 4; ch++)
+    for (int ch = 0; ch < 4; ch++)
 #pragma unroll
-    for (int q = 0; q < TK / 4; q++) {
-      const int k = ch * TK + lk + 4 * q;
-      const bool kin = k < t.K;
-      ra[ch][q] = (kin && li < t.M) ? A[li + (long long)k * t.ld] : (T)0;
-      rb[ch][q] = (kin && li < t.N) ? (binv ? B[li * 64 + k] : B[li + (long long)k * t.ld]) : (T)0;
-    }
+      for (int q = 0; q < TK / 4; q++) {
+        const int kc = min(k0 + ch * TK + lk + 4 * q, t.K - 1);
+        ra[ch][q] = A[lam + (long long)kc * t.ld];
+        rb[ch][q] = binv ? B[lbn * 64 + kc] : B[lbn + (long long)kc * t.ld];
+      }
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++)
+#pragma unroll
+      for (int q = 0; q < TK / 4; q++) {
+        asm volatile("" : "+v"(ra[ch][q]));
+        asm volatile("" : "+v"(rb[ch][q]));
+      }
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++)
+#pragma unroll
+      for (int q = 0; q < TK / 4; q++) {
+        const int k = k0 + ch * TK + lk + 4 * q;
+        if (!(k < t.K && li < t.M)) ra[ch][q] = (T)0;
+        if (!(k < t.K && li < t.N)) rb[ch][q] = (T)0;
+      }
+  };
+  fetch4(0);
   const int c16 = lane & 15, g = lane >> 4;
   const int nch = (t.K + TK - 1) / TK;
+  for (int c0 = 0; c0 < nch; c0 += 4) {
 #pragma unroll
   for (int h = 0; h < 2; h++) {
-    if (2 * h >= nch) break;
-    if (h) __syncthreads();  // the first half's fragments are read
+    if (c0 + 2 * h >= nch) break;
+    if (c0 > 0 || h) __syncthreads();  // the previous chunks' fragments are read
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -1551,10 +1576,11 @@ __device__ __forceinline__ void gemm_tile_short(const GemmTask* __restrict__ tas
         As[c][li][lk + 4 * q] = ra[2 * h + c][q];
         Bs[c][li][lk + 4 * q] = rb[2 * h + c][q];
       }
+    if (h == 1 && c0 + 4 < nch) fetch4((c0 + 4) * TK);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-      if (2 * h + c >= nch) break;
+      if (c0 + 2 * h + c >= nch) break;
       T fa0[TK / 4], fa1[TK / 4], fb0[TK / 4], fb1[TK / 4];
 #pragma unroll
       for (int s2 = 0; s2 < TK / 4; s2++) {
@@ -1572,9 +1598,30 @@ __device__ __forceinline__ void gemm_tile_short(const GemmTask* __restrict__ tas
       }
     }
   }
+  }
 
+  // the C entries loaded together (clamped to the tile, dropped outside it)
   T* C = ((t.flags & 1) ? U : L) + t.c_off;
   const bool diag = (t.flags & 2) != 0;
+  T cv[2][2][4];
+  if (!binv) {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = min(wr + a * 16 + (lane & 15), t.M - 1);
+          const int j = min(wc + b * 16 + Mfma<T>::drow(lane, r), t.N - 1);
+          cv[a][b][r] = C[i + (long long)j * t.ldc];
+        }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) asm volatile("" : "+v"(cv[a][b][r]));
+  }
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -1585,7 +1632,7 @@ __device__ __forceinline__ void gemm_tile_short(const GemmTask* __restrict__ tas
         int j = wc + b * 16 + Mfma<T>::drow(lane, r);
         if (i < t.M && j < t.N && (!diag || i >= j)) {
           if (binv) C[i + (long long)j * t.ldc] = acc[a][b][r];
-          else C[i + (long long)j * t.ldc] -= acc[a][b][r];
+          else C[i + (long long)j * t.ldc] = cv[a][b][r] - acc[a][b][r];
         }
       }
 }
@@ -1600,8 +1647,9 @@ __global__ __launch_bounds__(256) void k_gemm_tile(const GemmTask* __restrict__ 
 {
   gemm_tile<T, 1>(tasks, Lsrc, L, U, Inv);
 }
-// Updates with K <= 64 (the trailing update of a front's last outer panel
-// when it is at most 64 columns wide).
+// The latency-shaped tile: updates with K <= 64 (the trailing update of a
+// front's last outer panel when it is at most 64 columns wide) and the main
+// chain's launches with few tiles.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_short(const GemmTask* __restrict__ tasks,
                                                     const T* Lsrc, T* L, T* U,
