@@ -1484,8 +1484,28 @@ __device__ __forceinline__ void gemm_tile(const GemmTask* __restrict__ tasks, co
     __syncthreads();
   }
 
+  // the C entries loaded together (clamped to the tile, dropped outside it)
   T* C = ((t.flags & 1) ? U : L) + t.c_off;
   const bool diag = (t.flags & 2) != 0;
+  T cv[2][2][4];
+  if (!binv) {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = min(wr + a * 16 + (lane & 15), t.M - 1);
+          const int j = min(wc + b * 16 + Mfma<T>::drow(lane, r), t.N - 1);
+          cv[a][b][r] = C[i + (long long)j * t.ldc];
+        }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) asm volatile("" : "+v"(cv[a][b][r]));
+  }
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -1496,7 +1516,7 @@ __device__ __forceinline__ void gemm_tile(const GemmTask* __restrict__ tasks, co
         int j = wc + b * 16 + Mfma<T>::drow(lane, r);
         if (i < t.M && j < t.N && (!diag || i >= j)) {
           if (binv) C[i + (long long)j * t.ldc] = acc[a][b][r];
-          else C[i + (long long)j * t.ldc] -= acc[a][b][r];
+          else C[i + (long long)j * t.ldc] = cv[a][b][r] - acc[a][b][r];
         }
       }
 }
@@ -1924,18 +1944,43 @@ __global__ __launch_bounds__(512) void k_gemm_tile128(const GemmTask* __restrict
     __syncthreads();
   }
 
+  // C -= acc, eight entries a lane at a time with their loads in flight
+  // together (clamped to the tile, dropped outside it; the empty asm keeps
+  // the compiler from sinking each load into its store's condition; eight
+  // keep the kernel within the 128 registers of four waves per SIMD)
   T* C = ((t.flags & 1) ? U : L) + t.c_off;
   const bool diag = (t.flags & 2) != 0;
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a0 = 0; a0 < 4; a0++) {
+    T cv[1][2][4];
 #pragma unroll
-    for (int b = 0; b < 2; b++)
+    for (int a = 0; a < 1; a++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int i = wr + a * 16 + (lane & 15);
-        const int j = wc + b * 16 + Mfma<T>::drow(lane, r);
-        if (i < t.M && j < t.N && (!diag || i >= j)) C[i + (long long)j * t.ldc] -= acc[a][b][r];
-      }
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = min(wr + (a0 + a) * 16 + (lane & 15), t.M - 1);
+          const int j = min(wc + b * 16 + Mfma<T>::drow(lane, r), t.N - 1);
+          cv[a][b][r] = C[i + (long long)j * t.ldc];
+        }
+#pragma unroll
+    for (int a = 0; a < 1; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) asm volatile("" : "+v"(cv[a][b][r]));
+#pragma unroll
+    for (int a = 0; a < 1; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = wr + (a0 + a) * 16 + (lane & 15);
+          const int j = wc + b * 16 + Mfma<T>::drow(lane, r);
+          if (i < t.M && j < t.N && (!diag || i >= j))
+            C[i + (long long)j * t.ldc] = cv[a][b][r] - acc[a0 + a][b][r];
+        }
+  }
 }
 
 // Same tile, BF16 inputs with FP32 accumulation (v_mfma_f32_16x16x32_bf16):
