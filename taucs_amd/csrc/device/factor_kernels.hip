@@ -1850,7 +1850,27 @@ __global__ __launch_bounds__(256) void k_panel_solve(const SolveTask* __restrict
         if (i < t.M && j < t.N) L[t.c_off + i + j * ld] = X[a][b][r];
       }
   if (t.Md == 0) return;
-  // Cd -= X X^T (both operands X, staged once into As)
+  // Cd -= X X^T (both operands X, staged once into As).  Cd's entries are
+  // loaded first, together (clamped to the block, dropped outside it), so
+  // that their round trip overlaps the product; no other workgroup of the
+  // launch writes this block.
+  T dv[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = min(wr + 16 * a + c16, t.Md - 1);
+        const int j = min(wc + 16 * b + Mfma<T>::drow(lane, r), t.Md - 1);
+        dv[a][b][r] = L[t.d_off + i + j * ld];
+      }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) asm volatile("" : "+v"(dv[a][b][r]));
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -1871,7 +1891,7 @@ __global__ __launch_bounds__(256) void k_panel_solve(const SolveTask* __restrict
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = wr + 16 * a + c16, j = wc + 16 * b + Mfma<T>::drow(lane, r);
-        if (i < t.Md && j < t.Md && i >= j) L[t.d_off + i + j * ld] -= acc[a][b][r];
+        if (i < t.Md && j < t.Md && i >= j) L[t.d_off + i + j * ld] = dv[a][b][r] - acc[a][b][r];
       }
   if (t.pf_front < 0) return;
   __syncthreads();  // the block's update is stored, and As read, before the POTRF
