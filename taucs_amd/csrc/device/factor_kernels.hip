@@ -1439,25 +1439,35 @@ __device__ __forceinline__ void gemm_tile(const GemmTask* __restrict__ tasks, co
       for (int r = 0; r < 4; r++) acc[a][b][r] = (T)0;
 
   const int li = tid & 63, lk = tid >> 6;  // loader: row li, k = lk + 4q
+  const int lam = min(li, t.M - 1), lbn = min(li, t.N - 1);
   T ra[TK / 4], rb[TK / 4];
+  // the chunk's loads, unconditional (entries outside the operands read a
+  // clamped in-range one), so that they stay in flight during the MFMAs;
+  // stash zeroes those entries as it stores the chunk into LDS (the empty
+  // asm there keeps the compiler from sinking a load into that select)
   auto fetch = [&](int kc) {
 #pragma unroll
     for (int q = 0; q < TK / 4; q++) {
-      const int k = kc + lk + 4 * q;
-      const bool kin = k < t.K;
-      ra[q] = (kin && li < t.M) ? A[li + (long long)k * t.ld] : (T)0;
-      rb[q] = (kin && li < t.N) ? (binv ? B[li * 64 + k] : B[li + (long long)k * t.ld]) : (T)0;
+      const int kcl = min(kc + lk + 4 * q, t.K - 1);
+      ra[q] = A[lam + (long long)kcl * t.ld];
+      rb[q] = binv ? B[lbn * 64 + kcl] : B[lbn + (long long)kcl * t.ld];
     }
   };
-  auto stash = [&](int buf) {
+  auto stash = [&](int buf, int kc) {
 #pragma unroll
     for (int q = 0; q < TK / 4; q++) {
-      As[buf][li][lk + 4 * q] = ra[q];
-      Bs[buf][li][lk + 4 * q] = rb[q];
+      asm volatile("" : "+v"(ra[q]));
+      asm volatile("" : "+v"(rb[q]));
+    }
+#pragma unroll
+    for (int q = 0; q < TK / 4; q++) {
+      const bool kin = kc + lk + 4 * q < t.K;
+      As[buf][li][lk + 4 * q] = (kin && li < t.M) ? ra[q] : (T)0;
+      Bs[buf][li][lk + 4 * q] = (kin && li < t.N) ? rb[q] : (T)0;
     }
   };
   fetch(0);
-  stash(0);
+  stash(0, 0);
   __syncthreads();
   const int c16 = lane & 15, g = lane >> 4;
   const int nch = (t.K + TK - 1) / TK;
@@ -1480,7 +1490,7 @@ __device__ __forceinline__ void gemm_tile(const GemmTask* __restrict__ tasks, co
       acc[1][1] = Mfma<T>::step(fb1[s2], fa1[s2], acc[1][1]);
     }
     if (NBUF == 1) __syncthreads();
-    if (ch + 1 < nch) stash(NBUF == 2 ? (buf ^ 1) : 0);
+    if (ch + 1 < nch) stash(NBUF == 2 ? (buf ^ 1) : 0, (ch + 1) * TK);
     __syncthreads();
   }
 
@@ -1554,8 +1564,9 @@ __device__ __forceinline__ void gemm_tile_short(const GemmTask* __restrict__ tas
   const int li = tid & 63, lk = tid >> 6;  // loader: row li, k = 16 ch + lk + 4q
   const int lam = min(li, t.M - 1), lbn = min(li, t.N - 1);
   T ra[4][TK / 4], rb[4][TK / 4];
-  // (the empty asm gives every loaded value an unconditional use, so the
-  // compiler cannot sink a load into the select below and wait for it there)
+  // the group's loads only (clamped, unconditional); the entries outside
+  // the operands are zeroed as they are stored into LDS, where the empty asm
+  // keeps the compiler from sinking a load into that select
   auto fetch4 = [&](int k0) {
 #pragma unroll
     for (int ch = 0; ch < 4; ch++)
@@ -1564,21 +1575,6 @@ __device__ __forceinline__ void gemm_tile_short(const GemmTask* __restrict__ tas
         const int kc = min(k0 + ch * TK + lk + 4 * q, t.K - 1);
         ra[ch][q] = A[lam + (long long)kc * t.ld];
         rb[ch][q] = binv ? B[lbn * 64 + kc] : B[lbn + (long long)kc * t.ld];
-      }
-#pragma unroll
-    for (int ch = 0; ch < 4; ch++)
-#pragma unroll
-      for (int q = 0; q < TK / 4; q++) {
-        asm volatile("" : "+v"(ra[ch][q]));
-        asm volatile("" : "+v"(rb[ch][q]));
-      }
-#pragma unroll
-    for (int ch = 0; ch < 4; ch++)
-#pragma unroll
-      for (int q = 0; q < TK / 4; q++) {
-        const int k = k0 + ch * TK + lk + 4 * q;
-        if (!(k < t.K && li < t.M)) ra[ch][q] = (T)0;
-        if (!(k < t.K && li < t.N)) rb[ch][q] = (T)0;
       }
   };
   fetch4(0);
@@ -1593,8 +1589,16 @@ __device__ __forceinline__ void gemm_tile_short(const GemmTask* __restrict__ tas
     for (int c = 0; c < 2; c++)
 #pragma unroll
       for (int q = 0; q < TK / 4; q++) {
-        As[c][li][lk + 4 * q] = ra[2 * h + c][q];
-        Bs[c][li][lk + 4 * q] = rb[2 * h + c][q];
+        asm volatile("" : "+v"(ra[2 * h + c][q]));
+        asm volatile("" : "+v"(rb[2 * h + c][q]));
+      }
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int q = 0; q < TK / 4; q++) {
+        const bool kin = (c0 + 2 * h + c) * TK + lk + 4 * q < t.K;
+        As[c][li][lk + 4 * q] = (kin && li < t.M) ? ra[2 * h + c][q] : (T)0;
+        Bs[c][li][lk + 4 * q] = (kin && li < t.N) ? rb[2 * h + c][q] : (T)0;
       }
     if (h == 1 && c0 + 4 < nch) fetch4((c0 + 4) * TK);
     __syncthreads();
